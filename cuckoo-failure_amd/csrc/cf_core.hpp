@@ -1,0 +1,356 @@
+// cf_core.hpp -- the per-thread building blocks of the suffix-table engine.
+//
+// Everything here is written once and compiled twice: by hipcc for gfx950 (the product kernels
+// in cf_kernels.hip call these functions with the thread index and put a workgroup barrier
+// between phases) and by a host compiler for a CPU emulation that runs the same phases thread by
+// thread (a debugging build, never a product path).
+//
+// Arithmetic. Residues are doubles holding integers. A prime p < 2^50 leaves three spare bits of
+// the 53-bit mantissa, so sums up to 8p are exact. The product a*b is split exactly into
+// h + l with one FMA (h = fl(ab), l = fma(a, b, -h)); q = rint(h / p) is the nearest integer to
+// ab/p up to a small error, and r = fma(-q, p, h) + l = ab - qp is computed without rounding.
+// With |a|, |b| bounded the result is bounded (see mulred), so values are kept "balanced"
+// (roughly in (-p, p)) and only brought to [0, p) at the very end.
+//
+// The algorithm is the one-table suffix-table recurrence of the reference
+// (include/cuckoo/single_table.hpp:80-96, 178-229; fast/cuckoo_fast.cu:145-243) in a rescaled
+// form that saves multiplications:
+//   Vh[p][w] = V[p][w] / (w! p!)          (stored suffix table)
+//   Sh[v][q] = S[v][q] / (q! v!)          (stored shared inner sums)
+// Step v, stage 1 (column v of Sh from row v of Vh):
+//   a[w]  = Vh[v][w] / (v-1-w)!                    = U[v][w] / v!,   w < v
+//   c     = a (*) k,  k[t] = 1/t (1 <= t < nm)     cyclic, length L >= nm
+//   Sh[v'][v] = c[v'] / (v'-v)!                    v <= v' < nm
+// Step v, stage 2 (column v of Vh from row v of Sh), g = g(v):
+//   b[q]  = Sh[v][q] * v! g^(v-q) / (v-q)!         1 <= q <= v
+//   c     = b (*) k
+//   Vh[p][v] = (g^v/v! * 1/p - c[p mod L]) * g^(p-v)/v! / (p-v-1)!     v < p <= nm
+// and the query sum of row v (fast/cuckoo_fast.cu:306-310) is sum_w a[w] / (M - w) = out[v] / v!.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define CF_HD __host__ __device__ __forceinline__
+#else
+#define CF_HD inline
+#endif
+
+namespace cf {
+
+// ------------------------------------------------------------------------------------------
+// Exact modular arithmetic on doubles
+// ------------------------------------------------------------------------------------------
+struct Mod {
+    double p;     // the prime, p < 2^50
+    double pinv;  // fl(1 / p)
+};
+
+// a*b reduced: returns ab - qp with q = rint(fl(ab) * pinv). For |a| <= A p, |b| <= B p the result
+// satisfies |r| <= p/2 + 3 * 2^-53 |ab| <= (1/2 + 3 A B / 8) p, and it is exact whenever that bound
+// is below 2^53 (the partial h - qp is an integer of about the same size).
+#ifdef CF_EMU_CHECK
+// CPU emulation only: verify every product exactly (a b - r divisible by p, |r| below 2^52).
+void emu_check_mulred(double a, double b, double r, const Mod& m);
+#define CF_CHECK_MULRED(a, b, r, m) emu_check_mulred(a, b, r, m)
+#else
+#define CF_CHECK_MULRED(a, b, r, m)
+#endif
+
+CF_HD double mulred(double a, double b, const Mod& m) {
+    const double h = a * b;
+    const double l = __builtin_fma(a, b, -h);
+    const double q = __builtin_rint(h * m.pinv);
+    const double r = __builtin_fma(-q, m.p, h);
+    CF_CHECK_MULRED(a, b, r + l, m);
+    return r + l;
+}
+// x - rint(x/p) p: for |x| <= 16p the result lies in [-p/2 - 4, p/2 + 4].
+CF_HD double reduce(double x, const Mod& m) {
+    const double q = __builtin_rint(x * m.pinv);
+    return __builtin_fma(-q, m.p, x);
+}
+// The canonical residue in [0, p).
+CF_HD double canon(double x, const Mod& m) {
+    double r = reduce(x, m);
+    if (r < 0) r += m.p;
+    if (r >= m.p) r -= m.p;
+    return r;
+}
+CF_HD double powmod(double b, uint64_t e, const Mod& m) {
+    double r = 1.0;
+    b = reduce(b, m);
+    while (e) {
+        if (e & 1) r = mulred(r, b, m);
+        b = mulred(b, b, m);
+        e >>= 1;
+    }
+    return reduce(r, m);
+}
+
+// ------------------------------------------------------------------------------------------
+// Small DFTs in registers (natural order in and out). W[i] = w16^i for i < 8 (w16 a primitive
+// 16th root of unity, W[0] unused). Radix-2 decimation in frequency; the bit-reversed result is
+// renamed back to natural order at compile time. Inputs must satisfy |x| <= p/2 + 4: after four
+// levels every entry is below 16 (p/2 + 4) < 2^53.
+// ------------------------------------------------------------------------------------------
+template <int H>
+CF_HD void dif_level(double (&x)[16], int base, const double* W, const Mod& m) {
+    // pairs (i, i + H) inside the block [base, base + 2H); twiddle w_{2H}^i = W[(8 / H) i]
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const double a = x[base + i], b = x[base + i + H];
+        x[base + i] = a + b;
+        const double d = a - b;
+        x[base + i + H] = i ? mulred(d, W[(8 / H) * i], m) : d;
+    }
+}
+
+CF_HD int bitrev4(int k) { return ((k & 1) << 3) | ((k & 2) << 1) | ((k & 4) >> 1) | ((k & 8) >> 3); }
+
+// Radix-R DFT on the 16 / R consecutive groups of R entries (R = 2, 4, 8, 16).
+template <int R>
+CF_HD void dft_groups(double (&x)[16], const double* W, const Mod& m) {
+#pragma unroll
+    for (int g = 0; g < 16; g += R) {
+        if (R >= 16) dif_level<8>(x, g, W, m);
+        if (R >= 8)
+#pragma unroll
+            for (int b = 0; b < R; b += 8) dif_level<4>(x, g + b, W, m);
+        if (R >= 4)
+#pragma unroll
+            for (int b = 0; b < R; b += 4) dif_level<2>(x, g + b, W, m);
+#pragma unroll
+        for (int b = 0; b < R; b += 2) dif_level<1>(x, g + b, W, m);
+    }
+    // the entry at offset j of a group holds frequency bitrev_R(j)
+    double y[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) y[i] = x[i];
+    constexpr int sh = R == 16 ? 0 : R == 8 ? 1 : R == 4 ? 2 : 3;
+#pragma unroll
+    for (int g = 0; g < 16; g += R)
+#pragma unroll
+        for (int k = 0; k < R; ++k) x[g + k] = y[g + (bitrev4(k) >> sh)];
+}
+
+CF_HD void reduce16(double (&x)[16], const Mod& m) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = reduce(x[i], m);
+}
+
+// x[k] *= tau^k for k = 1..15 with tau, tau^2, tau^4, tau^8 given (reduced): 11 products build the
+// other powers, 15 apply them.
+CF_HD void twiddle_chain(double (&x)[16], const double* b, int stride, const Mod& m) {
+    const double t1 = b[0], t2 = b[stride], t4 = b[2 * stride], t8 = b[3 * stride];
+    const double t3 = mulred(t1, t2, m), t5 = mulred(t4, t1, m), t6 = mulred(t4, t2, m);
+    const double t7 = mulred(t4, t3, m);
+    const double t9 = mulred(t8, t1, m), t10 = mulred(t8, t2, m), t11 = mulred(t8, t3, m);
+    const double t12 = mulred(t8, t4, m), t13 = mulred(t8, t5, m), t14 = mulred(t8, t6, m);
+    const double t15 = mulred(t8, t7, m);
+    x[1] = mulred(x[1], t1, m), x[2] = mulred(x[2], t2, m), x[3] = mulred(x[3], t3, m);
+    x[4] = mulred(x[4], t4, m), x[5] = mulred(x[5], t5, m), x[6] = mulred(x[6], t6, m);
+    x[7] = mulred(x[7], t7, m), x[8] = mulred(x[8], t8, m), x[9] = mulred(x[9], t9, m);
+    x[10] = mulred(x[10], t10, m), x[11] = mulred(x[11], t11, m), x[12] = mulred(x[12], t12, m);
+    x[13] = mulred(x[13], t13, m), x[14] = mulred(x[14], t14, m), x[15] = mulred(x[15], t15, m);
+}
+
+// LDS position with one pad word per 16: spreads the strided accesses of the later rounds over
+// the banks.
+CF_HD int lpos(int P) { return P + (P >> 4); }
+
+// ------------------------------------------------------------------------------------------
+// Per-task context: pointers into the task's slice of device memory and its constants.
+// ------------------------------------------------------------------------------------------
+struct TaskDesc {
+    double p, pinv;
+    double root, rooti;  // primitive L-th root of unity and its inverse (balanced)
+    double linv;         // 1 / L
+    double W[8], Wi[8];  // w16^i and w16^-i, w16 = root^(L/16)
+    int64_t arena;       // offset (in doubles) of the task's memory
+    int nm;              // largest n among the task's queries: table rows 0..nm
+    int F1;              // factorial tables cover 0..F1-1
+    int q_off, q_cnt;    // the task's queries (sorted by n, largest first)
+    int slot_off;        // offset (in ints) of the tile slot table for this nm
+    int pool;            // tiles in each of the two pools
+    int NB, NS;          // 16-row blocks of the tables; staging rows (16 NB)
+    int NT;              // threads of the task's workgroup (L / 16)
+};
+
+struct QueryDesc {
+    int n, M;
+    int64_t out;  // index into the result array
+};
+
+// Layout of a task's arena (doubles), in this order.
+struct Arena {
+    double *Vpool, *Spool;  // pool tiles of 16 x 16 entries, row-major
+    double *Vst, *Sst;      // staging: 16 columns x NS rows, column-major
+    double *fact, *ifact, *inv;
+    double* qtot;  // per query: Horner accumulator, then Q = g(M)
+    double* hat;   // kernel spectrum times 1/L, in the final round's order (L entries)
+    double* tw0;   // round-0 twiddle bases: [8][T], tau^(1,2,4,8) forward then inverse
+};
+
+CF_HD int64_t arena_doubles(int pool, int NS, int F1, int q_cnt, int L) {
+    return (int64_t)pool * 512 + 32 * (int64_t)NS + 3 * (int64_t)F1 + 2 * (int64_t)q_cnt + L + L / 2;
+}
+
+CF_HD Arena arena_of(double* base, const TaskDesc& t) {
+    Arena a;
+    a.Vpool = base + t.arena;
+    a.Spool = a.Vpool + (int64_t)t.pool * 256;
+    a.Vst = a.Spool + (int64_t)t.pool * 256;
+    a.Sst = a.Vst + 16 * (int64_t)t.NS;
+    a.fact = a.Sst + 16 * (int64_t)t.NS;
+    a.ifact = a.fact + t.F1;
+    a.inv = a.ifact + t.F1;
+    a.qtot = a.inv + t.F1;
+    a.hat = a.qtot + 2 * (int64_t)t.q_cnt;
+    a.tw0 = a.hat + (int64_t)16 * t.NT;
+    return a;
+}
+
+// Entry (r, c) of a table with rows r and columns c < r: tile (r/16, c/16) from the pool when the
+// column block has been flushed (c/16 < r/16), else the staging column c%16.
+CF_HD double load_tab(const double* pool, const double* st, const int* slots, int NS, int r, int c) {
+    const int Y = r >> 4, X = c >> 4;
+    if (X < Y) return pool[(int64_t)slots[Y * (Y - 1) / 2 + X] * 256 + ((r & 15) << 4) + (c & 15)];
+    return st[(int64_t)(c & 15) * NS + r];
+}
+
+// ------------------------------------------------------------------------------------------
+// The NTT plan for L = 2^LOG2L with T = L/16 threads, 16 entries per thread.
+// Round r < NTW is a radix-16 round on groups of stride S_r = L / 16^(r+1): DFT16 over the group
+// then the twiddle w_{16 S_r}^(n2 k) (four-step decomposition, in place). The final round is a
+// radix-RF DFT on consecutive entries (RF = 16 when LOG2L is a multiple of 4). The forward output
+// is in digit-reversed order, which the pointwise product and the mirrored inverse never undo.
+// ------------------------------------------------------------------------------------------
+template <int LOG2L>
+struct Plan {
+    static constexpr int L = 1 << LOG2L;
+    static constexpr int T = L >> 4;
+    static constexpr int REM = LOG2L & 3;
+    static constexpr int NTW = REM ? (LOG2L >> 2) : (LOG2L >> 2) - 1;  // twiddled radix-16 rounds
+    static constexpr int RF = REM ? (1 << REM) : 16;                   // radix of the final round
+    static constexpr int LDSW = L + (L >> 4);                          // padded buffer (doubles)
+    static constexpr int lgS(int r) { return LOG2L - 4 * (r + 1); }
+    // twiddle tables of rounds 1..NTW-1 (16 S_r entries each), forward then inverse
+    static constexpr int tw_off(int r) {
+        int o = 0;
+        for (int k = 1; k < r; ++k) o += 16 << lgS(k);
+        return o;
+    }
+    static constexpr int TWN = tw_off(NTW);  // entries per direction
+    static constexpr int HI = (L >> 6) + 2;  // power-table entries g^(64 j), j <= nm/64
+};
+
+// Shared (LDS) state of a workgroup.
+template <int LOG2L>
+struct Shared {
+    using P = Plan<LOG2L>;
+    double buf[P::LDSW];
+    double tw[2 * P::TWN + 2];
+    double lo_pre[64], lo_post[64], hi[P::HI];
+    double red[P::T / 64 * 8 + 8];  // query partial sums: [wave][8]
+    double scal[8];                 // broadcast scalars
+};
+
+// Per-thread state.
+struct Thread {
+    double x[16];
+};
+
+template <int LOG2L>
+struct Engine {
+    using P = Plan<LOG2L>;
+    static constexpr int L = P::L, T = P::T;
+
+    // ---- forward round 0 on registers (entries n1 T + t), result stored to LDS ----
+    static CF_HD void fwd0(Thread& th, Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m, const double* tw0) {
+        reduce16(th.x, m);
+        dft_groups<16>(th.x, d.W, m);
+        twiddle_chain(th.x, tw0 + t, T, m);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sh.buf[lpos(k * T + t)] = th.x[k];
+    }
+    // ---- forward round r (1 <= r < NTW) ----
+    static CF_HD void fwdr(int r, Thread& th, Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m) {
+        const int lg = P::lgS(r), S = 1 << lg;
+        const int b = t >> lg, n2 = t & (S - 1), base = (b << (lg + 4)) + n2;
+        double(&x)[16] = th.x;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = reduce(sh.buf[lpos(base + k * S)], m);
+        dft_groups<16>(x, d.W, m);
+        const double* tw = sh.tw + P::tw_off(r);
+#pragma unroll
+        for (int k = 1; k < 16; ++k) x[k] = mulred(x[k], tw[k * S + n2], m);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sh.buf[lpos(base + k * S)] = x[k];
+    }
+    // ---- final round: forward radix-RF, then either keep (kernel spectrum) or multiply by the
+    //      spectrum and run the inverse radix-RF ----
+    static CF_HD void final_round(bool make_hat, Thread& th, Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m,
+                                  double* hat) {
+        double(&x)[16] = th.x;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = reduce(sh.buf[lpos(16 * t + i)], m);
+        dft_groups<P::RF>(x, d.W, m);
+        double* h = hat + 16 * t;
+        if (make_hat) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) h[i] = reduce(mulred(x[i], d.linv, m), m);
+            return;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = reduce(mulred(x[i], h[i], m), m);
+        dft_groups<P::RF>(x, d.Wi, m);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) sh.buf[lpos(16 * t + i)] = x[i];
+    }
+    // ---- inverse round r (1 <= r < NTW): twiddle w^-(n2 k), then inverse DFT16 ----
+    static CF_HD void invr(int r, Thread& th, Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m) {
+        const int lg = P::lgS(r), S = 1 << lg;
+        const int b = t >> lg, n2 = t & (S - 1), base = (b << (lg + 4)) + n2;
+        double(&x)[16] = th.x;
+        const double* tw = sh.tw + P::TWN + P::tw_off(r);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const double y = sh.buf[lpos(base + k * S)];
+            x[k] = reduce(k ? mulred(y, tw[k * S + n2], m) : y, m);
+        }
+        dft_groups<16>(x, d.Wi, m);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sh.buf[lpos(base + k * S)] = x[k];
+    }
+    // ---- inverse round 0: LDS -> registers (entries n1 T + t, natural order) ----
+    static CF_HD void inv0(Thread& th, Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m, const double* tw0) {
+        double(&x)[16] = th.x;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = sh.buf[lpos(k * T + t)];
+        twiddle_chain(x, tw0 + 4 * T + t, T, m);
+        reduce16(x, m);
+        dft_groups<16>(x, d.Wi, m);
+    }
+
+    // ---- prologue pieces ----
+    // round-0 twiddle bases and the LDS tables of rounds >= 1 (entries idx = t, t + T, ...)
+    static CF_HD void twiddles(Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m, double* tw0) {
+        double f = powmod(d.root, (uint64_t)t, m), i = powmod(d.rooti, (uint64_t)t, m);
+        for (int k = 0; k < 4; ++k) {
+            tw0[k * T + t] = f, tw0[(4 + k) * T + t] = i;
+            f = reduce(mulred(f, f, m), m), i = reduce(mulred(i, i, m), m);
+        }
+        for (int r = 1; r < P::NTW; ++r) {
+            const int lg = P::lgS(r), S = 1 << lg;
+            for (int idx = t; idx < 16 * S; idx += T) {
+                const int k = idx >> lg, n2 = idx & (S - 1);
+                const uint64_t e = ((uint64_t)n2 * (uint64_t)k << (4 * r)) & (L - 1);
+                sh.tw[P::tw_off(r) + idx] = powmod(d.root, e, m);
+                sh.tw[P::TWN + P::tw_off(r) + idx] = powmod(d.rooti, e, m);
+            }
+        }
+    }
+};
+
+}  // namespace cf

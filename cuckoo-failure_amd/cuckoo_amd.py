@@ -1,0 +1,156 @@
+"""Python binding of the MI355X one-table failure-count engine (include/cuckoo_amd.h).
+
+The engine is the HIP library lib/libcuckoo_amd.so built by `make` in this directory. There is
+no fallback: if the library is missing or no HIP device is present, the calls raise.
+
+    from cuckoo_amd import Engine, prime_range, need
+    with Engine() as e:
+        res = e.run([(16384, 32768)], prime_range(0, 8), d=3)   # res[b][j] = N_bad mod p_b
+
+Host-side functions (prime_range, need, transform_length, ntt_primes32) mirror the reference's
+host logic (fast/cuckoo_fast.cu:423-472, include/cuckoo/modint.hpp:183-196) and need no device.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcuckoo_amd.so")
+CLI_FAST = os.path.join(HERE, "bin", "cuckoo_fast")
+CLI_FAIL = os.path.join(HERE, "bin", "cuckoo_fail")
+
+CF_OK, CF_E_DEVICE, CF_E_ARGS, CF_E_NOMEM, CF_E_PRIMES, CF_E_UNSUPPORTED = 0, 1, 2, 3, 4, 5
+
+_lib = None
+u64 = ctypes.c_uint64
+
+
+class CfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (code {code})")
+        self.code = code
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double), ("launches", ctypes.c_int),
+                ("tasks", ctypes.c_int), ("batches", ctypes.c_int), ("log2_length", ctypes.c_int),
+                ("device_bytes", ctypes.c_size_t)]
+
+
+def lib():
+    """The loaded engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is missing: build it with `make -C {HERE}`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.cf_open.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.cf_close.argtypes = [ctypes.c_void_p]
+        L.cf_close.restype = None
+        L.cf_strerror.restype = ctypes.c_char_p
+        L.cf_strerror.argtypes = [ctypes.c_int]
+        L.cf_last_error.restype = ctypes.c_char_p
+        L.cf_last_error.argtypes = [ctypes.c_void_p]
+        L.cf_last_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+        L.cf_device_name.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.cf_run_fused.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t, ctypes.POINTER(u64), ctypes.c_size_t,
+                                   ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(u64)]
+        L.cf_max_batch.restype = ctypes.c_long
+        L.cf_max_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
+        L.cf_mem_per_prime.restype = ctypes.c_size_t
+        L.cf_mem_per_prime.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
+        L.cf_prime_range.restype = ctypes.c_long
+        L.cf_prime_range.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_long, ctypes.POINTER(u64)]
+        L.cf_need.restype = ctypes.c_long
+        L.cf_need.argtypes = [ctypes.c_int] * 4
+        L.cf_transform_length.restype = ctypes.c_size_t
+        L.cf_transform_length.argtypes = [ctypes.c_size_t]
+        L.cf_ntt_primes32.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
+        L.cf_engine_log2_length.argtypes = [ctypes.c_int, u64]
+        _lib = L
+    return _lib
+
+
+# ---- host logic ----
+
+def prime_range(first, count, prime_bits=50):
+    out = (u64 * max(count, 1))()
+    k = lib().cf_prime_range(prime_bits, first, count, out)
+    if k < 0:
+        raise ValueError("bad prime range")
+    return [out[i] for i in range(k)]
+
+
+def need(n, M, d, prime_bits=50):
+    return lib().cf_need(n, M, d, prime_bits)
+
+
+def transform_length(nmax):
+    return lib().cf_transform_length(nmax)
+
+
+def ntt_primes32(count, min_pow2):
+    out = (ctypes.c_uint32 * max(count, 1))()
+    if lib().cf_ntt_primes32(count, min_pow2, out) != count:
+        raise RuntimeError("not enough NTT-friendly primes")
+    return list(out)[:count]
+
+
+def mem_per_prime(nmax, Mmax, J=1):
+    return lib().cf_mem_per_prime(nmax, Mmax, J)
+
+
+# ---- the device engine ----
+
+class Engine:
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        rc = lib().cf_open(device, ctypes.byref(self._h))
+        if rc != CF_OK:
+            raise CfError(rc, f"cannot open HIP device {device}: {lib().cf_strerror(rc).decode()}")
+
+    def close(self):
+        if self._h:
+            lib().cf_close(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_name(self):
+        buf = ctypes.create_string_buffer(256)
+        lib().cf_device_name(self._h, buf, 256)
+        return buf.value.decode()
+
+    def run(self, queries, primes, d, model=0, mask=None):
+        """N_bad(n, M) mod p for every prime and query: returns res[b][j] (0 where mask[b][j] is 0)."""
+        J, np_ = len(queries), len(primes)
+        n = (ctypes.c_int32 * max(J, 1))(*[q[0] for q in queries])
+        M = (ctypes.c_int32 * max(J, 1))(*[q[1] for q in queries])
+        P = (u64 * max(np_, 1))(*primes)
+        res = (u64 * max(np_ * J, 1))()
+        mk = None
+        if mask is not None:
+            flat = [1 if mask[b][j] else 0 for b in range(np_) for j in range(J)]
+            mk = (ctypes.c_uint8 * max(len(flat), 1))(*flat)
+        rc = lib().cf_run_fused(self._h, d, model, n, M, J, P, np_, mk, res)
+        if rc != CF_OK:
+            raise CfError(rc, lib().cf_last_error(self._h).decode())
+        return [[res[b * J + j] for j in range(J)] for b in range(np_)]
+
+    def stats(self):
+        s = Stats()
+        lib().cf_last_stats(self._h, ctypes.byref(s))
+        return s
+
+    def max_batch(self, nmax, Mmax, J=1):
+        return lib().cf_max_batch(self._h, nmax, Mmax, J)

@@ -1,0 +1,99 @@
+/*
+ * cuckoo_amd.h -- C ABI of the MI355X (gfx950) one-table failure-count engine.
+ *
+ * The reference (zcy05331/Cuckoo-Failure) has no library API for this path: its seams are the
+ * in-source back end of fast/cuckoo_fast.cu (launch / dev_alloc / to_dev / from_dev / dev_sync /
+ * dev_free_bytes at :325-386, convolve at :633-646), the v-loop and query reduction driven from
+ * main (:787-900), and the 32-bit library calls build_one_table_suffix_ntt(nmax, model) and
+ * OneTableSuffix::nbad(n, M) (include/cuckoo/single_table.hpp:97-120, 178-229). Each entry point
+ * below names the reference code it replaces.
+ *
+ * Conventions: plain C types; caller-owned host buffers; the engine owns all device memory.
+ * Residues cross the ABI in canonical form [0, p). Integer return codes map to the reference's
+ * exit codes (CF_E_ARGS -> 2, CF_E_NOMEM -> 3, CF_E_PRIMES -> 4, CF_E_DEVICE -> 1); no exception
+ * crosses the ABI. One engine per host thread.
+ */
+#ifndef CUCKOO_AMD_H
+#define CUCKOO_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CF_OK 0
+#define CF_E_DEVICE 1      /* HIP runtime or kernel failure (cuda_check, fast/cuckoo_fast.cu:329-334) */
+#define CF_E_ARGS 2        /* invalid arguments (fast/cuckoo_fast.cu:654-695) */
+#define CF_E_NOMEM 3       /* not enough device memory for one prime (fast/cuckoo_fast.cu:751-753) */
+#define CF_E_PRIMES 4      /* prime list exhausted (fast/cuckoo_fast.cu:729-732) */
+#define CF_E_UNSUPPORTED 5 /* a shape this build does not run (transform length beyond the LDS kernels,
+                              or p - 1 without the needed power of two) */
+
+typedef struct cf_engine cf_engine;
+
+/* Timing and size of the last cf_run_fused call. kernel_ms is measured with HIP events on the
+ * engine's stream around the task kernels only; wall_ms covers the whole call. */
+typedef struct {
+    double kernel_ms;
+    double wall_ms;
+    int launches;
+    int tasks;
+    int batches;
+    int log2_length; /* transform length of the last launch */
+    size_t device_bytes;
+} cf_stats;
+
+/* Replaces the CUDA back end's device set-up (fast/cuckoo_fast.cu:328-359): selects a HIP device
+ * and creates the engine's stream. */
+int cf_open(int device, cf_engine** out);
+void cf_close(cf_engine* e);
+const char* cf_strerror(int code);
+/* Human-readable detail of the last error on this engine. */
+const char* cf_last_error(const cf_engine* e);
+int cf_last_stats(const cf_engine* e, cf_stats* out);
+/* Device name and architecture, e.g. "AMD Instinct MI355X gfx950". */
+int cf_device_name(const cf_engine* e, char* buf, size_t len);
+
+/* Builds the suffix table for each prime and answers the queries on it: the reference's per-batch
+ * work of fast/cuckoo_fast.cu:787-896 (host precompute, kernel spectrum, the v-loop of stage-1 /
+ * stage-2 convolutions, the query reduction and the Horner finish), and equally the 32-bit path's
+ * build_one_table_suffix_ntt + nbad (include/cuckoo/single_table.hpp:103-119, 178-229).
+ *   d        candidate slots per item (d >= 1)
+ *   model    0: uniform d-subset, g(v) = C(v, d), Q = C(M, d); 1: d draws with replacement,
+ *            g(v) = v^d, Q = M^d (single_table.hpp:16-23)
+ *   n, M     J queries (1 <= n < M)
+ *   primes   np odd primes p < 2^50 (each with M < p)
+ *   mask     np x J bytes, row-major; query j is evaluated for prime b iff mask[b J + j] != 0.
+ *            NULL means all.
+ *   res      np x J canonical residues N_bad(n_j, M_j) mod p_b; 0 where the mask is 0.
+ * Each prime's table has as many rows as the largest n it is asked for. Returns CF_OK or an error
+ * code; cf_last_error() has the detail. */
+int cf_run_fused(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, size_t J,
+                 const uint64_t* primes, size_t np, const uint8_t* mask, uint64_t* res);
+
+/* How many primes of a table with nmax rows and factorials up to Mmax, answering J queries, fit
+ * in the device memory now free (the reference's batch sizing, fast/cuckoo_fast.cu:742-750). */
+long cf_max_batch(cf_engine* e, int nmax, int Mmax, size_t J);
+/* Device bytes one such prime takes. */
+size_t cf_mem_per_prime(int nmax, int Mmax, size_t J);
+
+/* ---- host logic of the reference, exported for drivers and tests (no device needed) ---- */
+/* Primes number first .. first+count-1 of the fixed list: the i-th largest p = c 2^20 + 1 below
+ * 2^prime_bits (fast/cuckoo_fast.cu:423-432). Returns how many were written to out. */
+long cf_prime_range(int prime_bits, long first, long count, uint64_t* out);
+/* Number of primes whose product exceeds Q^n, Q = C(M, d) (fast/cuckoo_fast.cu:452-459). */
+long cf_need(int n, int M, int d, int prime_bits);
+/* max(16, next_pow2(nmax)) (fast/cuckoo_fast.cu:465-472). */
+size_t cf_transform_length(size_t nmax);
+/* count 32-bit primes with 2-adic valuation k = 31 .. min_pow2, largest first
+ * (include/cuckoo/modint.hpp:183-196). Returns count, or -1 if the list runs out. */
+int cf_ntt_primes32(int count, int min_pow2, uint32_t* out);
+/* log2 of the engine's transform length for nmax rows modulo p, or -1 if this build cannot run
+ * that shape. */
+int cf_engine_log2_length(int nmax, uint64_t p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ by running the reference's own programs.
+
+The programs are the reference's C++ sources compiled unmodified by `make -C oracle ref`
+(into oracle/_ref/, never committed). Nothing from the reference is imported or copied: only
+the programs' outputs are kept, as data.
+
+    python tests/golden/make_golden.py            # everything except the n = 65536 prime
+    python tests/golden/make_golden.py --big      # also n = 65536 (about 32 GB RAM, ~3 min)
+"""
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(REPO, "oracle", "_ref")
+FAST = os.path.join(REF, "cuckoo_fast_ifma")
+if not os.path.exists(FAST) or "avx512ifma" not in open("/proc/cpuinfo").read():
+    FAST = os.path.join(REF, "cuckoo_fast_cpu")
+FAIL = os.path.join(REF, "cuckoo_fail")
+CHECK = os.path.join(REF, "check")
+
+
+def run(argv, cwd=None):
+    r = subprocess.run(argv, capture_output=True, text=True, cwd=cwd)
+    return r.returncode, r.stdout, r.stderr
+
+
+def strip_fast_stderr(s):
+    # progress lines carry wall-clock times; keep only their structure
+    return re.sub(r"build done at [0-9.]+ s, queries at [0-9.]+ s", "build done at T s, queries at T s", s)
+
+
+def fast_run(d, queries, first, count, extra=(), out_init=None):
+    with tempfile.TemporaryDirectory() as td:
+        qf = os.path.join(td, "q.txt")
+        with open(qf, "w") as f:
+            f.write("".join(f"{n} {M}\n" for n, M in queries))
+        argv = [FAST, str(d), qf, str(first), str(count), *extra]
+        outf = None
+        if out_init is not None:
+            outf = os.path.join(td, "out.txt")
+            with open(outf, "w") as f:
+                f.write(out_init)
+            argv += ["--out", outf]
+        rc, so, se = run(argv)
+        rec = {"d": d, "queries": queries, "first": first, "count": count, "extra": list(extra),
+               "rc": rc, "stdout": so, "stderr": strip_fast_stderr(se)}
+        if outf is not None:
+            rec["out_init"] = out_init
+            rec["out_final"] = open(outf).read()
+        return rec
+
+
+def fast_cases():
+    """Residue lines of the 50-bit engine over many shapes: every n class around the powers of
+    two (the c[nmax] -> c[0] wrap when L = nmax), d = 1..5, multi-query files with different
+    need, and prime ranges that start past 0."""
+    cases = []
+    for d in (1, 2, 3, 4, 5):
+        for n in (8, 9, 15, 16, 17, 31, 32, 33, 64, 100, 127, 128, 129):
+            cases.append((d, [(n, n + 1), (n, 2 * n + 3)], 0, 2))
+    cases.append((3, [(37, 80), (64, 65), (200, 400), (512, 634)], 0, 3))
+    cases.append((3, [(1000, 1500), (777, 2000), (37, 38)], 0, 2))
+    cases.append((2, [(255, 256), (256, 257), (257, 258), (1024, 1025), (1023, 3000)], 5, 3))
+    cases.append((4, [(2048, 4096)], 0, 2))
+    cases.append((3, [(4096, 8192)], 0, 2))
+    # query lists whose need straddles the range: lines only for i < need(n, M)
+    cases.append((3, [(10, 11), (8, 200), (40, 41), (12, 13)], 0, 12))
+    cases.append((2, [(30, 31), (9, 10)], 3, 9))
+    # full prime sets, for exact reconstruction by CRT
+    cases.append((3, [(64, 65)], 0, 60))
+    cases.append((3, [(200, 400)], 0, 97))
+    cases.append((1, [(12, 20)], 0, 8))
+    cases.append((2, [(6, 7), (8, 9)], 0, 3))
+    with ThreadPoolExecutor(max_workers=os.cpu_count()) as ex:
+        return list(ex.map(lambda c: fast_run(*c), cases))
+
+
+def fast_errors():
+    """Exit codes and messages of the argument/input checks (cuckoo_fast.cu:654-695, 724-732)."""
+    recs = []
+    recs.append(fast_run(3, [(7, 20)], 0, 2))  # max n < 8
+    recs.append(fast_run(3, [(20, 20)], 0, 2))  # M <= n
+    recs.append(fast_run(3, [(20, 40)], 0, 2, extra=("--bogus",)))
+    recs.append(fast_run(3, [(20, 40)], 500, 2))  # range past need: nothing to do
+    recs.append(fast_run(0, [(20, 40)], 0, 2))  # d < 1
+    rc, so, se = run([FAST, "3"])
+    recs.append({"argv": ["3"], "rc": rc, "stdout": so, "stderr": se})
+    rc, so, se = run([FAST, "3", "/nonexistent/q.txt", "0", "1"])
+    recs.append({"argv": ["3", "/nonexistent/q.txt", "0", "1"], "rc": rc, "stdout": so, "stderr": se})
+    return recs
+
+
+def fast_resume():
+    """--out resume semantics (cuckoo_fast.cu:697-727): complete, partial, foreign and truncated
+    records in a pre-existing file."""
+    recs = []
+    q = [(20, 40), (12, 13)]
+    full = fast_run(3, q, 0, 6)["stdout"].splitlines(keepends=True)
+    recs.append(fast_run(3, q, 0, 6, out_init=""))  # fresh file
+    recs.append(fast_run(3, q, 0, 6, out_init="".join(full[:4])))  # two primes complete
+    recs.append(fast_run(3, q, 0, 6, out_init="".join(full[:3])))  # prime 1 partial
+    recs.append(fast_run(3, q, 0, 6, out_init="".join(full)))  # all done
+    recs.append(fast_run(3, q, 0, 6, out_init="".join(full[:4]) + "2 1125899846025217 20"))  # truncated tail
+    recs.append(fast_run(3, q, 0, 6, out_init="0 5 9 9 9\n0 5 9 9 9\n1 5 9 9 9\n"))  # foreign (n, M)
+    recs.append(fast_run(3, q, 2, 3, out_init="".join(full[:6])))  # range starting past done primes
+    return recs
+
+
+def fast_big(n, M, first, count):
+    with ThreadPoolExecutor(max_workers=min(count, os.cpu_count())) as ex:
+        recs = list(ex.map(lambda i: fast_run(3, [(n, M)], i, 1), range(first, first + count)))
+    return {"d": 3, "queries": [[n, M]], "first": first, "count": count,
+            "stdout": "".join(r["stdout"] for r in recs)}
+
+
+def fail_cases():
+    """The reference's 32-bit CLI (src/cuckoo_fail.cpp), ntt algorithm."""
+    args = [
+        ["single", "-n", "200", "-M", "400", "-d", "3", "--algo", "ntt", "--exact"],
+        ["single", "-n", "20", "-M", "40", "-d", "3"],
+        ["single", "-n", "20", "-M", "40", "-d", "3", "--exact"],
+        ["single", "-n", "6", "-M", "7", "-d", "2", "--exact"],
+        ["single", "-n", "1", "-M", "2", "-d", "1", "--exact"],
+        ["single", "-n", "5", "-M", "6", "-d", "1", "--exact"],
+        ["single", "-n", "12", "-M", "20", "-d", "1", "--exact"],
+        ["single", "-n", "31", "-M", "32", "-d", "2", "--exact"],
+        ["single", "-n", "60", "-M", "100", "-d", "3", "--exact"],
+        ["single", "-n", "64", "-M", "65", "-d", "3", "--exact"],
+        ["single", "-n", "100", "-M", "150", "-d", "4", "--exact"],
+        ["single", "-n", "128", "-M", "1000", "-d", "2", "--exact"],
+        ["single", "-n", "300", "-M", "301", "-d", "5"],
+        ["single", "-n", "512", "-M", "634", "-d", "3", "--exact"],
+        ["single", "-n", "30", "-M", "50", "-d", "2", "--replacement", "--exact"],
+        ["single", "-n", "25", "-M", "40", "-d", "3", "--replacement", "--exact"],
+        ["single", "-n", "200", "-M", "400", "-d", "3", "--mod", "998244353"],
+        ["single", "-n", "200", "-M", "400", "-d", "3", "--mod", "1004535809"],
+        ["single", "-n", "50", "-M", "51", "-d", "3", "--mod", "469762049"],
+        ["single", "-n", "10", "-M", "10", "-d", "3"],
+        ["single", "-n", "10", "-M", "20", "-d", "3", "--algo", "bogus"],
+        ["single", "-n", "10", "-M", "20", "-d", "3", "--mod", "1000"],
+        ["bogus"],
+        [],
+    ]
+    with ThreadPoolExecutor(max_workers=os.cpu_count()) as ex:
+        outs = list(ex.map(lambda a: run([FAIL, *a]), args))
+    recs = []
+    for a, (rc, so, se) in zip(args, outs):
+        so = re.sub(r"time: [0-9.]+ s", "time: T s", so)
+        so = re.sub(r"\([0-9.]+ s\)", "(T s)", so)
+        recs.append({"args": a, "rc": rc, "stdout": so, "stderr": se})
+    return recs
+
+
+def main():
+    os.makedirs(HERE, exist_ok=True)
+    rc, so, _ = run([CHECK])
+    assert rc == 0, "reference check program failed"
+    with open(os.path.join(HERE, "check_ref.txt"), "w") as f:
+        f.write(re.sub(r"\([0-9.]+ s\)", "(T s)", so))
+    with open(os.path.join(HERE, "fast_cases.json"), "w") as f:
+        json.dump(fast_cases(), f, indent=0)
+    with open(os.path.join(HERE, "fast_errors.json"), "w") as f:
+        json.dump(fast_errors(), f, indent=0)
+    with open(os.path.join(HERE, "fast_resume.json"), "w") as f:
+        json.dump(fast_resume(), f, indent=0)
+    with open(os.path.join(HERE, "fail_cases.json"), "w") as f:
+        json.dump(fail_cases(), f, indent=0)
+    with open(os.path.join(HERE, "fast_16384.json"), "w") as f:
+        json.dump(fast_big(16384, 32768, 0, 8), f, indent=0)
+    if "--big" in sys.argv:
+        with open(os.path.join(HERE, "fast_65536.json"), "w") as f:
+            json.dump(fast_big(65536, 131072, 0, 1), f, indent=0)
+
+
+if __name__ == "__main__":
+    main()

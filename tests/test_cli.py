@@ -1,0 +1,113 @@
+"""The two drivers against the reference's own outputs: argument and input errors, resume
+decisions that stop before any device work (CPU), and full runs (GPU)."""
+import os
+import subprocess
+import tempfile
+
+import pytest
+
+import cuckoo_amd as cf
+from golden_data import load, residue_lines
+
+
+def run_fast(d, queries, first, count, extra=(), out_init=None, argv=None):
+    with tempfile.TemporaryDirectory() as td:
+        if argv is None:
+            qf = os.path.join(td, "q.txt")
+            with open(qf, "w") as f:
+                f.write("".join(f"{n} {M}\n" for n, M in queries))
+            argv = [str(d), qf, str(first), str(count), *extra]
+        outf = None
+        if out_init is not None:
+            outf = os.path.join(td, "out.txt")
+            with open(outf, "w") as f:
+                f.write(out_init)
+            argv = argv + ["--out", outf]
+        r = subprocess.run([cf.CLI_FAST, *argv], capture_output=True, text=True, timeout=600)
+        final = open(outf).read() if outf else None
+        return r.returncode, r.stdout, r.stderr, final
+
+
+@pytest.mark.parametrize("idx", range(7))
+def test_fast_argument_errors(idx):
+    rec = load("fast_errors.json")[idx]
+    if "argv" in rec:
+        argv = rec["argv"]
+        rc, so, se, _ = run_fast(None, None, None, None, argv=argv)
+    else:
+        rc, so, se, _ = run_fast(rec["d"], rec["queries"], rec["first"], rec["count"], extra=rec["extra"])
+    assert rc == rec["rc"]
+    assert so == rec["stdout"]
+    assert se.strip() == rec["stderr"].strip()
+
+
+def test_fast_resume_all_done_stops_before_the_device():
+    rec = load("fast_resume.json")[3]
+    rc, so, se, final = run_fast(rec["d"], rec["queries"], rec["first"], rec["count"], out_init=rec["out_init"])
+    assert rc == rec["rc"] == 0
+    assert se.strip() == rec["stderr"].strip()
+    assert final == rec["out_final"]
+
+
+@pytest.mark.parametrize("idx", [19, 20, 21, 22, 23])
+def test_fail_argument_errors(idx):
+    rec = load("fail_cases.json")[idx]
+    r = subprocess.run([cf.CLI_FAIL, *rec["args"]], capture_output=True, text=True, timeout=60)
+    assert r.returncode == rec["rc"] == 2
+    assert r.stderr.splitlines()[0] == rec["stderr"].splitlines()[0]
+    assert r.stdout.startswith("usage:")
+
+
+def test_fail_rejects_what_this_build_does_not_run():
+    for args in (["separate", "-n", "5", "-m", "3,3"], ["single", "-n", "5", "-M", "9", "-d", "2", "--algo", "cubic"]):
+        r = subprocess.run([cf.CLI_FAIL, *args], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "not available" in r.stderr
+
+
+# ------------------------------------------------------------------ GPU
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("idx", [0, 1, 2, 4, 5, 6])
+def test_fast_resume_scenarios(idx):
+    rec = load("fast_resume.json")[idx]
+    rc, so, se, final = run_fast(rec["d"], rec["queries"], rec["first"], rec["count"], out_init=rec["out_init"])
+    assert rc == rec["rc"] == 0, se
+    assert final == rec["out_final"]
+    assert so == ""
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("idx", range(0, 80, 7))
+def test_fast_cli_matches_reference_lines(idx):
+    recs = load("fast_cases.json")
+    if idx >= len(recs):
+        pytest.skip("fewer cases")
+    rec = recs[idx]
+    rc, so, se, _ = run_fast(rec["d"], rec["queries"], rec["first"], rec["count"])
+    assert rc == rec["rc"] == 0, se
+    assert so == rec["stdout"]
+
+
+@pytest.mark.gpu
+def test_fast_cli_batches_do_not_change_output():
+    rec = next(r for r in load("fast_cases.json") if r["queries"] == [[10, 11], [8, 200], [40, 41], [12, 13]])
+    for b in ("1", "3", "50"):
+        rc, so, se, _ = run_fast(rec["d"], rec["queries"], rec["first"], rec["count"], extra=("--batch", b))
+        assert rc == 0, se
+        assert so == rec["stdout"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("idx", range(19))
+def test_fail_cli_matches_reference(idx):
+    rec = load("fail_cases.json")[idx]
+    r = subprocess.run([cf.CLI_FAIL, *rec["args"]], capture_output=True, text=True, timeout=600)
+    assert r.returncode == rec["rc"], r.stderr
+    want = [l for l in rec["stdout"].splitlines() if "time:" not in l and "(T s)" not in l]
+    got = [l for l in r.stdout.splitlines() if "time:" not in l and not l.startswith("N_bad mod")]
+    assert got == want
+    for wl, gl in zip(rec["stdout"].splitlines(), r.stdout.splitlines()):
+        if wl.startswith("primes:"):
+            assert gl.split("time:")[0] == wl.split("time:")[0]
+        if wl.startswith("N_bad mod"):
+            assert gl.split("(")[0] == wl.split("(")[0]
