@@ -14,18 +14,23 @@
 //
 // The algorithm is the one-table suffix-table recurrence of the reference
 // (include/cuckoo/single_table.hpp:80-96, 178-229; fast/cuckoo_fast.cu:145-243) in a rescaled
-// form that saves multiplications:
-//   Vh[p][w] = V[p][w] / (w! p!)          (stored suffix table)
-//   Sh[v][q] = S[v][q] / (q! v!)          (stored shared inner sums)
-// Step v, stage 1 (column v of Sh from row v of Vh):
-//   a[w]  = Vh[v][w] / (v-1-w)!                    = U[v][w] / v!,   w < v
+// form in which every factorial that depends on both indices of an entry is applied once, when
+// the entry is written, and never looked up when it is read:
+//   V2[p][w] = V[p][w] / (w! p! (p-1-w)!)     (stored suffix table)
+//   S2[v][q] = S[v][q] / (q! v! (v-q)!)       (stored shared inner sums)
+// Step v, stage 1 (column v of S2 from row v of V2):
+//   a[w]  = V2[v][w]                               = U[v][w] / v!,   w < v
 //   c     = a (*) k,  k[t] = 1/t (1 <= t < nm)     cyclic, length L >= nm
-//   Sh[v'][v] = c[v'] / (v'-v)!                    v <= v' < nm
-// Step v, stage 2 (column v of Vh from row v of Sh), g = g(v):
-//   b[q]  = Sh[v][q] * v! g^(v-q) / (v-q)!         1 <= q <= v
+//   S2[v'][v] = c[v'] / (v'-v)!^2                  v <= v' < nm
+// Step v, stage 2 (column v of V2 from row v of S2), g = g(v):
+//   b[q]  = S2[v][q] * v! g^(v-q)                  1 <= q <= v
 //   c     = b (*) k
-//   Vh[p][v] = (g^v/v! * 1/p - c[p mod L]) * g^(p-v)/v! / (p-v-1)!     v < p <= nm
+//   V2[p][v] = (g^v/v! * 1/p - c[p mod L]) * g^(p-v)/v! / (p-v-1)!^2     v < p <= nm
 // and the query sum of row v (fast/cuckoo_fast.cu:306-310) is sum_w a[w] / (M - w) = out[v] / v!.
+// The table actually stored is P2[p][w] = p V2[p][w]: with B = sum_q b[q] and c' = (q b) (*) k,
+// p c[p] = B + c'[p] (since p/(p-q) = 1 + q/(p-q)), so
+//   P2[p][v] = (g^v/v! - B - c'[p mod L]) * g^(p-v)/v! / (p-v-1)!^2
+// needs no per-entry 1/p, and the row read at step v is scaled back by the one number 1/v.
 #pragma once
 #include <stdint.h>
 
@@ -171,7 +176,8 @@ struct TaskDesc {
     int nm;              // largest n among the task's queries: table rows 0..nm
     int F1;              // factorial tables cover 0..F1-1
     int q_off, q_cnt;    // the task's queries (sorted by n, largest first)
-    int slot_off;        // offset (in ints) of the tile slot table for this nm
+    int slot_off;        // offset (in ints) of the segment table for this nm
+    int NSEG;            // segments per row block
     int pool;            // tiles in each of the two pools
     int NB, NS;          // 16-row blocks of the tables; staging rows (16 NB)
     int NT;              // threads of the task's workgroup (L / 16)
@@ -211,12 +217,26 @@ CF_HD Arena arena_of(double* base, const TaskDesc& t) {
     return a;
 }
 
-// Entry (r, c) of a table with rows r and columns c < r: tile (r/16, c/16) from the pool when the
-// column block has been flushed (c/16 < r/16), else the staging column c%16.
-CF_HD double load_tab(const double* pool, const double* st, const int* slots, int NS, int r, int c) {
+// A double at a byte offset below 4 GiB from a base pointer that is the same for the whole
+// workgroup: the per-entry address stays one 32-bit register.
+CF_HD double ldo(const double* base, uint32_t off) { return *(const double*)((const char*)base + off); }
+CF_HD uint32_t ix8(int i) { return (uint32_t)i << 3; }
+
+// Byte offset of entry (r, c) (c < r) of a table from its pool base: tile (r/16, c/16) (its tile
+// number given) when the column block has been flushed, else the staging column c%16, which
+// starts st_off bytes after the pool base.
+CF_HD uint32_t tab_off(uint32_t st_off, int slot, int NS, int r, int c) {
     const int Y = r >> 4, X = c >> 4;
-    if (X < Y) return pool[(int64_t)slots[Y * (Y - 1) / 2 + X] * 256 + ((r & 15) << 4) + (c & 15)];
-    return st[(int64_t)(c & 15) * NS + r];
+    return X < Y ? ((uint32_t)slot << 11) + ((uint32_t)(r & 15) << 7) + ((uint32_t)(c & 15) << 3)
+                 : st_off + (((uint32_t)(c & 15) * (uint32_t)NS + (uint32_t)r) << 3);
+}
+
+// Address of entry (r, c) of a table with rows r and columns c < r: tile (r/16, c/16) of the pool
+// (slot given) when the column block has been flushed (c/16 < r/16), else the staging column c%16.
+// Branch-free, so a phase can issue all its loads before waiting for any.
+CF_HD const double* tab_addr(const double* pool, const double* st, int slot, int NS, int r, int c) {
+    const int Y = r >> 4, X = c >> 4;
+    return X < Y ? pool + (int64_t)slot * 256 + ((r & 15) << 4) + (c & 15) : st + (int64_t)(c & 15) * NS + r;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -243,6 +263,7 @@ struct Plan {
     }
     static constexpr int TWN = tw_off(NTW);  // entries per direction
     static constexpr int HI = (L >> 6) + 2;  // power-table entries g^(64 j), j <= nm/64
+    static constexpr int NSEG = ((L + 1 + 15) / 16 + 15) / 16;  // segments per row block, nm <= L
 };
 
 // Shared (LDS) state of a workgroup.
@@ -253,12 +274,16 @@ struct Shared {
     double tw[2 * P::TWN + 2];
     double lo_pre[64], lo_post[64], hi[P::HI];
     double red[P::T / 64 * 8 + 8];  // query partial sums: [wave][8]
+    double red2[P::T / 64 + 1];     // per-wave sums of the stage-2 input
     double scal[8];                 // broadcast scalars
+    int segrow[P::NSEG + 1];        // segment slots of the current row block
 };
 
 // Per-thread state.
 struct Thread {
-    double x[16];
+    double x[16];   // the thread's 16 transform entries
+    double pa[16];  // data fetched one phase ahead of its use: a table row, the factorials of a
+                    // stage output, or the kernel spectrum for the final round (never two at once)
 };
 
 template <int LOG2L>
@@ -290,20 +315,22 @@ struct Engine {
     }
     // ---- final round: forward radix-RF, then either keep (kernel spectrum) or multiply by the
     //      spectrum and run the inverse radix-RF ----
+    // make_hat: store the spectrum (times 1/L) to hat; otherwise multiply by the spectrum entries
+    // already fetched into th.pa.
     static CF_HD void final_round(bool make_hat, Thread& th, Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m,
                                   double* hat) {
         double(&x)[16] = th.x;
 #pragma unroll
         for (int i = 0; i < 16; ++i) x[i] = reduce(sh.buf[lpos(16 * t + i)], m);
         dft_groups<P::RF>(x, d.W, m);
-        double* h = hat + 16 * t;
         if (make_hat) {
+            double* h = hat + 16 * t;
 #pragma unroll
             for (int i = 0; i < 16; ++i) h[i] = reduce(mulred(x[i], d.linv, m), m);
             return;
         }
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = reduce(mulred(x[i], h[i], m), m);
+        for (int i = 0; i < 16; ++i) x[i] = reduce(mulred(x[i], th.pa[i], m), m);
         dft_groups<P::RF>(x, d.Wi, m);
 #pragma unroll
         for (int i = 0; i < 16; ++i) sh.buf[lpos(16 * t + i)] = x[i];

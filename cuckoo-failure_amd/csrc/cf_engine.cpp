@@ -240,7 +240,7 @@ int cf_run_fused(cf_engine* e, int d, int model, const int32_t* n, const int32_t
             auto so = slot_off.find(tp.nm);
             if (so == slot_off.end()) {
                 so = slot_off.emplace(tp.nm, (int)slots.size()).first;
-                slots.insert(slots.end(), sp.slots.begin(), sp.slots.end());
+                slots.insert(slots.end(), sp.segs.begin(), sp.segs.end());
             }
             t.arena = arena_off;
             t.nm = tp.nm;
@@ -250,6 +250,7 @@ int cf_run_fused(cf_engine* e, int d, int model, const int32_t* n, const int32_t
             t.slot_off = so->second;
             t.pool = sp.pool;
             t.NB = sp.NB;
+            t.NSEG = sp.NSEG;
             t.NS = sp.NS;
             t.NT = (1 << lg) / 16;
             for (int j : tp.queries) qd.push_back(QueryDesc{n[j], M[j], (int64_t)(tp.b * J + j)});

@@ -131,22 +131,24 @@ SlotPlan plan_slots(int nm) {
     sp.nm = nm;
     sp.NB = (nm + 1 + 15) / 16;
     sp.NS = 16 * sp.NB;
-    const int NB = sp.NB;
-    sp.slots.assign((size_t)NB * (NB - 1) / 2 + 1, -1);
-    std::vector<int> free_slots;
-    int pool = 0;
+    sp.NSEG = (sp.NB + 15) / 16;
+    const int NB = sp.NB, NSEG = sp.NSEG;
+    sp.segs.assign((size_t)NB * NSEG, -1);
+    std::vector<int> free_segs;
+    int nseg = 0;
     const int nflush = nm >> 4;  // column block X is flushed at the end of step 16X + 15 <= nm - 1
     for (int X = 0; X < nflush; ++X) {
-        // row block X was read during steps 16X .. 16X + 15: its tiles are dead
-        for (int Xp = 0; Xp < X; ++Xp) free_slots.push_back(sp.slots[(size_t)X * (X - 1) / 2 + Xp]);
+        // row block X was read during steps 16X .. 16X + 15: its segments are dead
+        for (int s = 0; 16 * s < X; ++s) free_segs.push_back(sp.segs[(size_t)X * NSEG + s]);
+        if (X % 16) continue;
         for (int Y = X + 1; Y < NB; ++Y) {
-            int s;
-            if (!free_slots.empty()) s = free_slots.back(), free_slots.pop_back();
-            else s = pool++;
-            sp.slots[(size_t)Y * (Y - 1) / 2 + X] = s;
+            int g;
+            if (!free_segs.empty()) g = free_segs.back(), free_segs.pop_back();
+            else g = nseg++;
+            sp.segs[(size_t)Y * NSEG + X / 16] = g;
         }
     }
-    sp.pool = std::max(pool, 1);
+    sp.pool = 16 * std::max(nseg, 1);
     return sp;
 }
 

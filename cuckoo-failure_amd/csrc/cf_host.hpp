@@ -31,14 +31,19 @@ int engine_log2_length(int nm, uint64_t p);
 // Constants of one prime for the device (roots, 1/L, the DFT16 roots), balanced doubles.
 void task_constants(uint64_t p, int log2L, TaskDesc& d);
 
-// Static assignment of the 16 x 16 tiles of one table to pool slots. Tile (Y, X), X < Y, holds
-// rows 16Y..16Y+15 and columns 16X..16X+15. It is written when column block X is flushed (end of
-// step 16X + 15) and dead once row block Y has been read (end of step 16Y + 15); slots of dead
-// tiles are reused (greedy, last freed first), so the pool holds about nm^2/512 tiles instead of
-// nm^2/512 * 2 for the whole triangle.
+// Static assignment of the suffix-table tiles to pool memory. A tile holds 16 rows x 16 columns
+// (row-major). Tile (Y, X), X < Y, holds rows 16Y..16Y+15 and columns 16X..16X+15; it is written
+// when column block X is flushed (end of step 16X + 15) and dead once row block Y has been read
+// (end of step 16Y + 15). Tiles are grouped into segments of 16 consecutive column blocks of one
+// row block: segment (Y, s) holds tiles (Y, 16s .. 16s+15) contiguously (32 KB), is allocated
+// when its first tile is written and freed with its row block. Freed segments are reused (last
+// freed first), so the pool holds about nm^2/4 entries per table instead of nm^2/2, a row of the
+// table lies in at most NSEG segments, and the segment numbers of one row block (NSEG ints) fit
+// in LDS.
 struct SlotPlan {
-    int nm = 0, NB = 0, NS = 0, pool = 0;
-    std::vector<int> slots;  // index Y (Y - 1) / 2 + X
+    int nm = 0, NB = 0, NS = 0, NSEG = 0;
+    int pool = 0;            // tiles (16 per segment)
+    std::vector<int> segs;   // segment slot of (Y, s) at Y * NSEG + s, -1 if never allocated
 };
 SlotPlan plan_slots(int nm);
 

@@ -105,13 +105,57 @@ struct Task {
     }
 
     // ---------------- per step ----------------
-    static CF_HD void stage1_pre(Thread& th, int t, int v, const Ctx& c) {
+    // Every stage function below is branch-free over its 16 entries: indices are clamped to a
+    // valid entry and the result is selected afterwards, so all loads of a phase are in flight
+    // together instead of one round trip per entry.
+
+    // Tile number of (Y = r/16, X) from the current row block's segment row (X < Y).
+    static CF_HD int tile_of(const Sh& sh, int X) { return sh.segrow[X >> 4] * 16 + (X & 15); }
+    // Segment row of row block Y into LDS.
+    static CF_HD void load_segrow(Sh& sh, int t, int Y, const Ctx& c) {
+        for (int s = t; s < c.d->NSEG; s += T) sh.segrow[s] = c.slots[Y * c.d->NSEG + s];
+    }
+    // Entries (r, e) of a table for this thread's e in [lo, hi), as raw loads into dst (other
+    // entries get the value at a clamped, valid address and must be masked by the consumer).
+    // Row block r/16 must be the one whose segment row is in LDS.
+    static CF_HD void fetch_row(double (&dst)[16], const double* pool, const double* st, const Sh& sh, int t, int r, int lo,
+                                int hi, const Ctx& c) {
+        const int Y = r >> 4;
+        const uint32_t st_off = (uint32_t)((st - pool) << 3);
+        const int NS = c.d->NS;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = k * T + t, ce = (e >= lo && e < hi) ? e : 0, X = ce >> 4;
+            const int tile = X < Y ? tile_of(sh, X) : 0;
+            dst[k] = ldo(pool, tab_off(st_off, tile, NS, r, ce));
+        }
+    }
+    static CF_HD void fetch_hat(Thread& th, int t, const Ctx& c) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) th.pa[i] = c.a.hat[16 * t + i];
+    }
+    // 1/(e - v - off)! for the stage outputs: off = 0 for stage 1 (rows v <= e < nm), off = 1 for
+    // stage 2 (rows v < p <= nm, entry 0 standing for p = L).
+    static CF_HD void fetch_ifact(Thread& th, int t, int v, int off, const Ctx& c) {
+        const int nm = c.d->nm;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = k * T + t, p = (off && e == 0) ? L : e;
+            const bool ok = off ? (p > v && p <= nm) : (e >= v && e < nm);
+            th.pa[k] = ldo(c.a.ifact, ix8(ok ? p - v - off : 0));
+        }
+    }
+    // Stage-1 input a[w] = P2[v][w] / v = V2[v][w] for w < v: columns w < v-1 from th.pa (fetched
+    // ahead, or now when fetch_now), column v-1 from the LDS hand-over of the previous step.
+    static CF_HD void stage1_pre(Thread& th, const Sh& sh, int t, int v, bool fetch_now, const Ctx& c) {
+        if (fetch_now) fetch_row(th.pa, c.a.Vpool, c.a.Vst, sh, t, v, 0, v - 1, c);
+        const double iv = c.a.inv[v], hand = sh.scal[1];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int e = k * T + t;
-            double a = 0.0;
-            if (e < v) a = mulred(load_tab(c.a.Vpool, c.a.Vst, c.slots, c.d->NS, v, e), c.a.ifact[v - 1 - e], c.m);
-            th.x[k] = a;
+            const double x = e < v - 1 ? th.pa[k] : hand;
+            const double a = mulred(x, iv, c.m);
+            th.x[k] = e < v ? a : 0.0;
         }
     }
     // Partial sums of the queries j0 .. j0+cnt-1 over this thread's entries of row v.
@@ -121,10 +165,15 @@ struct Task {
             double s = 0.0;
             if (jj < cnt) {
                 const int M = c.q[c.d->q_off + j0 + jj].M;
+                double iv[16];
 #pragma unroll
                 for (int k = 0; k < 16; ++k) {
                     const int e = k * T + t;
-                    if (e < v) s += mulred(th.x[k], c.a.inv[M - e], c.m);
+                    iv[k] = ldo(c.a.inv, ix8(M - (e < v ? e : 0)));
+                }
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    s += mulred(th.x[k], iv[k], c.m);  // th.x[k] = 0 for e >= v
                     if (k == 7) s = reduce(s, c.m);
                 }
             }
@@ -139,13 +188,31 @@ struct Task {
         const double tot = c.a.qtot[2 * j], Q = c.a.qtot[2 * j + 1];
         c.a.qtot[2 * j] = reduce(mulred(tot, Q, c.m) + mulred(coef, s, c.m), c.m);
     }
-    static CF_HD void stage1_post(const Thread& th, int t, int v, const Ctx& c) {
+    // Thread t < cnt finishes query j0 + t from the per-wave partial sums.
+    static CF_HD void query_finish(const Sh& sh, int t, int v, int j0, int cnt, const Ctx& c) {
+        if (t < cnt) {
+            double s = 0.0;
+            for (int w = 0; w < NW; ++w) s += sh.red[w * 8 + t];  // NW <= 16 values below p/2 + 4
+            query_update(t, v, j0, reduce(s, c.m), c);
+        }
+    }
+    // Stage-1 output S2[e][v] = c[e] / (e-v)!^2 for v <= e < nm into the staging column v % 16,
+    // with the factorials in th.pa (fetched ahead). Then row v of S2 is fetched into th.pa for the
+    // stage-2 input of the next phase; its entry (v, v) is the value this thread just computed.
+    static CF_HD void stage1_post(Thread& th, const Sh& sh, int t, int v, const Ctx& c) {
         double* col = c.a.Sst + (int64_t)(v & 15) * c.d->NS;
+        const int nm = c.d->nm;
+        double y[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int e = k * T + t;
-            if (e >= v && e < c.d->nm) col[e] = mulred(th.x[k], c.a.ifact[e - v], c.m);
+            y[k] = mulred(th.x[k], mulred(th.pa[k], th.pa[k], c.m), c.m);
+            if (e >= v && e < nm) col[e] = y[k];
         }
+        fetch_row(th.pa, c.a.Spool, c.a.Sst, sh, t, v, 1, v, c);
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (k * T + t == v) th.pa[k] = y[k];
     }
     // Column v of a staging area set to zero on rows lo..hi-1.
     static CF_HD void zero_col(double* st, int t, int v, int lo, int hi, const Ctx& c) {
@@ -168,70 +235,113 @@ struct Task {
     }
     static CF_HD double gpow(const double* lo, const Sh& sh, int e, const Mod& m) { return mulred(sh.hi[e >> 6], lo[e & 63], m); }
 
-    static CF_HD void stage2_pre(Thread& th, const Sh& sh, int t, int v, const Ctx& c) {
+    // Stage-2 input: b[q] = S2[v][q] * v! g^(v-q) for 1 <= q <= v (row v of S2 in th.pa); the
+    // transform gets q b[q], and this thread's part of B = sum_q b[q] is returned.
+    static CF_HD double stage2_pre(Thread& th, const Sh& sh, int t, int v, const Ctx& c) {
+        double bsum = 0.0;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int e = k * T + t;
-            double b = 0.0;
-            if (e >= 1 && e <= v) {
-                const double x = load_tab(c.a.Spool, c.a.Sst, c.slots, c.d->NS, v, e);
-                const double f = mulred(gpow(sh.lo_pre, sh, v - e, c.m), c.a.ifact[v - e], c.m);
-                b = mulred(x, f, c.m);
-            }
-            th.x[k] = b;
+            const int e = k * T + t, ok = e >= 1 && e <= v, ce = ok ? e : v;
+            const double b = ok ? mulred(th.pa[k], gpow(sh.lo_pre, sh, v - ce, c.m), c.m) : 0.0;
+            bsum += b;
+            if ((k & 3) == 3) bsum = reduce(bsum, c.m);  // four terms below 4.1 p
+            th.x[k] = mulred(b, (double)e, c.m);
         }
+        return reduce(bsum, c.m);
     }
-    static CF_HD void stage2_post(const Thread& th, const Sh& sh, int t, int v, bool conv, const Ctx& c) {
+    // Stage-2 output P2[p][v] = (g^v/v! - B - c'[p]) g^(p-v)/v! / (p-v-1)!^2 for v < p <= nm into
+    // the staging column v % 16 (factorials in th.pa, B the block sum). The entry of row v+1 also
+    // goes to the LDS hand-over for the next step's stage-1 input.
+    static CF_HD void stage2_post(const Thread& th, Sh& sh, int t, int v, bool conv, double B, const Ctx& c) {
         double* col = c.a.Vst + (int64_t)(v & 15) * c.d->NS;
-        const double s1 = gpow(sh.lo_post, sh, v, c.m);  // g^v / v!
+        const int nm = c.d->nm;
+        const double s1 = gpow(sh.lo_post, sh, v, c.m) - B;  // g^v / v! - B
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int e = k * T + t;
-            const int p = e ? e : L;  // c[p mod L]: entry 0 stands for p = L
-            if (p > v && p <= c.d->nm) {
-                const int j = p - v;
-                const double f = mulred(gpow(sh.lo_post, sh, j, c.m), c.a.ifact[j - 1], c.m);
-                const double u = mulred(s1, c.a.inv[p], c.m) - (conv ? reduce(th.x[k], c.m) : 0.0);
-                col[p] = mulred(u, f, c.m);
+            const int e = k * T + t, p = e ? e : L;  // c'[p mod L]: entry 0 stands for p = L
+            const bool ok = p > v && p <= nm;
+            const int j = ok ? p - v : 1;
+            const double fj = mulred(gpow(sh.lo_post, sh, j, c.m), mulred(th.pa[k], th.pa[k], c.m), c.m);
+            const double u = s1 - (conv ? reduce(th.x[k], c.m) : 0.0);
+            const double y = mulred(u, fj, c.m);
+            if (ok) col[p] = y;
+            if (p == v + 1) sh.scal[1] = y;
+        }
+    }
+    // Staging column block X -> pool tiles (Y, X) for every row block Y > X, in two phases: the
+    // tile numbers go to LDS (the transform buffer is free here), then every thread moves 16
+    // entries per trip with all 16 loads in flight.
+    static CF_HD void flush_tiles(Sh& sh, int t, int X, const Ctx& c) {
+        int* tl = (int*)sh.buf;
+        const int NSEG = c.d->NSEG;
+        for (int j = t; j < c.d->NB - 1 - X; j += T) tl[j] = c.slots[(X + 1 + j) * NSEG + (X >> 4)] * 16 + (X & 15);
+    }
+    static CF_HD void flush(const Sh& sh, double* pool, const double* st, int t, int X, const Ctx& c) {
+        const int* tl = (const int*)sh.buf;
+        const int n = (c.d->NB - 1 - X) * 256;
+        const uint32_t NS = (uint32_t)c.d->NS, row0 = 16 * (uint32_t)(X + 1);
+        for (int base = 0; base < n; base += 16 * T) {
+            double y[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int idx = base + u * T + t, ci = idx < n ? idx : 0;
+                const uint32_t j = (uint32_t)ci >> 8, col = ((uint32_t)ci >> 4) & 15, r = (uint32_t)ci & 15;
+                y[u] = ldo(st, (col * NS + row0 + 16 * j + r) << 3);
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int idx = base + u * T + t;
+                if (idx < n) {
+                    const int col = (idx >> 4) & 15, r = idx & 15;
+                    pool[(int64_t)tl[idx >> 8] * 256 + (r << 4) + col] = y[u];
+                }
             }
         }
     }
-    // Staging column block X -> pool tiles (Y, X) for every row block Y > X.
-    static CF_HD void flush(double* pool, const double* st, int t, int X, const Ctx& c) {
-        const int n = (c.d->NB - 1 - X) * 256;
-        for (int idx = t; idx < n; idx += T) {
-            const int Y = X + 1 + (idx >> 8), col = (idx >> 4) & 15, r = idx & 15;
-            pool[(int64_t)c.slots[Y * (Y - 1) / 2 + X] * 256 + (r << 4) + col] = st[(int64_t)col * c.d->NS + 16 * Y + r];
-        }
-    }
 
-    // ---------------- the convolution: x <- x (*) k ----------------
+    // ---------------- the convolution, split at its first and last phase ----------------
+    // The caller fuses its own work into the first phase (before fwd0) and the last (after inv0).
+    // The middle phases also fetch ahead: the kernel spectrum for the final round, and the
+    // factorials 1/(e - v - off)! that the caller's last phase needs.
     template <class R>
-    static CF_HD void convolve(R& run, Sh& sh, const Ctx& c) {
-        run.phase([&](Thread& th, int t) { E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0); });
-        for (int r = 1; r < P::NTW; ++r) run.phase([&](Thread& th, int t) { E::fwdr(r, th, sh, t, *c.d, c.m); });
+    static CF_HD void conv_middle(R& run, Sh& sh, int v, int off, const Ctx& c) {
+        for (int r = 1; r < P::NTW; ++r)
+            run.phase([&](Thread& th, int t) {
+                if (r == P::NTW - 1) fetch_hat(th, t, c);
+                E::fwdr(r, th, sh, t, *c.d, c.m);
+            });
         run.phase([&](Thread& th, int t) { E::final_round(false, th, sh, t, *c.d, c.m, c.a.hat); });
-        for (int r = P::NTW - 1; r >= 1; --r) run.phase([&](Thread& th, int t) { E::invr(r, th, sh, t, *c.d, c.m); });
-        run.phase([&](Thread& th, int t) { E::inv0(th, sh, t, *c.d, c.m, c.a.tw0); });
+        for (int r = P::NTW - 1; r >= 1; --r)
+            run.phase([&](Thread& th, int t) {
+                if (r == 1) fetch_ifact(th, t, v, off, c);
+                E::invr(r, th, sh, t, *c.d, c.m);
+            });
     }
 
-    // Row v of Vh is in th.x (stage-1 input form); add its query terms.
-    template <class R>
-    static CF_HD void queries(R& run, Sh& sh, int v, const Ctx& c) {
+    static CF_HD int active_queries(int v, const Ctx& c) {
         // queries are sorted by n, largest first: the active ones (n >= v) are a prefix
         int nact = 0;
         while (nact < c.d->q_cnt && c.q[c.d->q_off + nact].n >= v) ++nact;
-        for (int j0 = 0; j0 < nact; j0 += 8) {
+        return nact;
+    }
+
+    // Row v of Vh is in th.x (stage-1 input form): add its query terms, in phases of their own.
+    template <class R>
+    static CF_HD void queries(R& run, Sh& sh, int v, int j_from, const Ctx& c) {
+        const int nact = active_queries(v, c);
+        for (int j0 = j_from; j0 < nact; j0 += 8) {
             const int cnt = nact - j0 < 8 ? nact - j0 : 8;
-            run.wave_reduce8(sh.red, [&](Thread& th, int t, double (&acc)[8]) { query_partial(th, t, v, j0, cnt, acc, c); }, c.m);
-            run.phase([&](Thread&, int t) {
-                if (t < cnt) {
-                    double s = 0.0;
-                    for (int w = 0; w < NW; ++w) s += sh.red[w * 8 + t];  // NW <= 16 values below p/2 + 4
-                    query_update(t, v, j0, reduce(s, c.m), c);
-                }
-            });
+            run.wave_reduce8(sh.red, [&](Thread& th, int t, double (&acc)[8]) {
+                query_partial(th, t, v, j0, cnt, acc, c);
+                return cnt;
+            }, c.m);
+            run.phase([&](Thread&, int t) { query_finish(sh, t, v, j0, cnt, c); });
         }
+    }
+
+    // Fetch ahead for step v + 1: its row of P2 except column v (which the hand-over carries).
+    static CF_HD void fetch_next(Thread& th, const Sh& sh, int t, int v, const Ctx& c) {
+        fetch_row(th.pa, c.a.Vpool, c.a.Vst, sh, t, v + 1, 0, v, c);
     }
 
     template <class R>
@@ -253,50 +363,106 @@ struct Task {
             inv_table(t, c);
             E::twiddles(sh, t, *c.d, c.m, c.a.tw0);
         });
+        // kernel spectrum (times 1/L) into the arena
         run.phase([&](Thread& th, int t) {
             query_init(t, c);
             kernel_input(th, t, c);
+            E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
         });
-        // kernel spectrum (times 1/L) into the arena
-        run.phase([&](Thread& th, int t) { E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0); });
         for (int r = 1; r < P::NTW; ++r) run.phase([&](Thread& th, int t) { E::fwdr(r, th, sh, t, *c.d, c.m); });
         run.phase([&](Thread& th, int t) { E::final_round(true, th, sh, t, *c.d, c.m, c.a.hat); });
 
         // ---- the v-loop ----
+        int segY = -1;       // row block whose segment row is in LDS
+        bool pref = false;  // th.pa holds row v of P2 (columns < v-1), fetched by the previous step
         for (int v = 1; v < nm; ++v) {
-            // stage 1: rows v <= v0 of V are zero, so column v of S is zero
-            if (v > c.v0) {
-                run.phase([&](Thread& th, int t) { stage1_pre(th, t, v, c); });
-                queries(run, sh, v, c);
-                convolve(run, sh, c);
-                run.phase([&](Thread& th, int t) { stage1_post(th, t, v, c); });
-            } else {
-                run.phase([&](Thread&, int t) { zero_col(c.a.Sst, t, v, v, nm, c); });
+            const bool s1 = v > c.v0;   // rows v <= v0 of V are zero, so column v of S is zero
+            const bool s2 = v >= c.v0;  // g(v) = 0 below v0, so column v of V is zero
+            const double g = s2 ? g_of(v, c) : 0.0;
+            if ((v >> 4) != segY && (s1 || s2)) {
+                segY = v >> 4;
+                run.phase([&](Thread&, int t) { load_segrow(sh, t, segY, c); });
             }
-            // stage 2: g(v) = 0 below v0, so column v of V is zero
-            if (v >= c.v0) {
-                const double g = g_of(v, c);
-                run.phase([&](Thread&, int t) { powtab(sh, t, v, g, c); });
+            if (s1) {
+                const int nact = active_queries(v, c);
+                const int cnt0 = nact < 8 ? nact : 8;
+                const bool now = !pref;
+                if (nact <= 8) {
+                    // stage-1 input, its query terms and the first forward round in one phase
+                    run.phase([&](Thread& th, int t) {
+                        stage1_pre(th, sh, t, v, now, c);
+                        if (cnt0) run.wave_partial8(sh.red, th, t, [&](Thread& th2, int t2, double (&acc)[8]) {
+                            query_partial(th2, t2, v, 0, cnt0, acc, c);
+                            return cnt0;
+                        }, c.m);
+                        E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
+                    });
+                } else {
+                    run.phase([&](Thread& th, int t) { stage1_pre(th, sh, t, v, now, c); });
+                    queries(run, sh, v, 0, c);
+                    run.phase([&](Thread& th, int t) { E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0); });
+                }
+                conv_middle(run, sh, v, 0, c);
+                // last inverse round, stage-1 output (then row v of S2 is fetched), query finish,
+                // power tables of stage 2
+                run.phase([&](Thread& th, int t) {
+                    if (nact <= 8 && cnt0) query_finish(sh, t, v, 0, cnt0, c);
+                    E::inv0(th, sh, t, *c.d, c.m, c.a.tw0);
+                    stage1_post(th, sh, t, v, c);
+                    if (s2) powtab(sh, t, v, g, c);
+                });
+            } else {
+                run.phase([&](Thread&, int t) {
+                    zero_col(c.a.Sst, t, v, v, nm, c);
+                    if (s2) powtab(sh, t, v, g, c);
+                });
+            }
+            // fetch ahead for step v + 1 unless a flush (and a new row block) comes in between
+            const bool next = s2 && (v & 15) != 15;
+            if (s2) {
                 const bool conv = v > c.v0;  // row v of S is zero unless v > v0
                 if (conv) {
-                    run.phase([&](Thread& th, int t) { stage2_pre(th, sh, t, v, c); });
-                    convolve(run, sh, c);
+                    run.phase([&](Thread& th, int t) {
+                        const double bpart = stage2_pre(th, sh, t, v, c);
+                        run.wave_partial1(sh.red2, th, t, bpart, c.m);
+                        E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
+                    });
+                    conv_middle(run, sh, v, 1, c);
+                    run.phase([&](Thread& th, int t) {
+                        double B = 0.0;
+                        for (int w = 0; w < NW; ++w) B += sh.red2[w];  // NW <= 16 values below p/2 + 4
+                        E::inv0(th, sh, t, *c.d, c.m, c.a.tw0);
+                        stage2_post(th, sh, t, v, true, reduce(B, c.m), c);
+                        if (next) fetch_next(th, sh, t, v, c);
+                    });
+                } else {
+                    run.phase([&](Thread& th, int t) {
+                        fetch_ifact(th, t, v, 1, c);
+                        stage2_post(th, sh, t, v, false, 0.0, c);
+                        if (next) fetch_next(th, sh, t, v, c);
+                    });
                 }
-                run.phase([&](Thread& th, int t) { stage2_post(th, sh, t, v, conv, c); });
             } else {
-                run.phase([&](Thread&, int t) { zero_col(c.a.Vst, t, v, v + 1, nm + 1, c); });
-            }
-            if ((v & 15) == 15) {
                 run.phase([&](Thread&, int t) {
-                    flush(c.a.Vpool, c.a.Vst, t, v >> 4, c);
-                    flush(c.a.Spool, c.a.Sst, t, v >> 4, c);
+                    zero_col(c.a.Vst, t, v, v + 1, nm + 1, c);
+                    if (t == 0) sh.scal[1] = 0.0;  // the hand-over entry (v+1, v) is zero too
+                });
+            }
+            pref = next;
+            if ((v & 15) == 15) {
+                run.phase([&](Thread&, int t) { flush_tiles(sh, t, v >> 4, c); });
+                run.phase([&](Thread&, int t) {
+                    flush(sh, c.a.Vpool, c.a.Vst, t, v >> 4, c);
+                    flush(sh, c.a.Spool, c.a.Sst, t, v >> 4, c);
                 });
             }
         }
         // row nm of V only feeds the queries with n = nm
         if (nm > c.v0) {
-            run.phase([&](Thread& th, int t) { stage1_pre(th, t, nm, c); });
-            queries(run, sh, nm, c);
+            if ((nm >> 4) != segY) run.phase([&](Thread&, int t) { load_segrow(sh, t, nm >> 4, c); });
+            const bool now = !pref;
+            run.phase([&](Thread& th, int t) { stage1_pre(th, sh, t, nm, now, c); });
+            queries(run, sh, nm, 0, c);
         }
     }
 
