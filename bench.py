@@ -13,7 +13,9 @@ average duration, measured with HIP events on the engine's stream, expressed as 
 FMA-equivalent TFLOP/s (one exact modmul = 6 FP64 VALU instructions) against the MI355X FP64
 vector peak of 78.6 TFLOP/s. `cpu_baseline` times the reference's own host build (compiled from
 its unmodified sources into oracle/_ref) on this machine's cores, on a bounded sample of the
-same workload.
+same workload. `traffic` is the HBM bytes of one launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
+passes scaled by the counters' readings for a plain copy of known size; it is read from the
+committed profile `profiles/k_tasks14_n16384_traffic.json` (null if that file is absent).
 """
 import argparse
 import json
@@ -64,6 +66,15 @@ def cpu_baseline(kind, procs=None):
     return {"value": procs / wall, "unit": "prime-residues/s", "cores": procs, "kind": "reference",
             "sample": f"{procs} primes (i=0..{procs - 1}) of n={N}, M={M}, d={D}, one process per prime, "
                       f"{'AVX-512 IFMA' if kind == 'ifma' else 'scalar'} host build, wall {wall:.1f} s"}
+
+
+def profiled_traffic():
+    path = os.path.join(REPO, "profiles", "k_tasks14_n16384_traffic.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main():
@@ -120,7 +131,7 @@ def main():
                    "primes": a.primes, "queries_per_prime": 1, "device": dev},
         "roofline": {"bound": "valu", "kernel": "k_tasks<14>", "achieved": round(achieved, 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                     "traffic": None, "launch_ms": round(per_launch_ms, 2),
+                     "traffic": profiled_traffic(), "launch_ms": round(per_launch_ms, 2),
                      "modmul_per_launch": mm_per_launch},
         "cpu_baseline": base,
     }

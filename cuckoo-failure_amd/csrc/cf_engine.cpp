@@ -218,13 +218,27 @@ int cf_run_fused(cf_engine* e, int d, int model, const int32_t* n, const int32_t
     size_t at = 0;
     float kernel_ms_total = 0;
     while (at < tasks.size()) {
-        // ---- one launch: tasks of one length that fit in memory together ----
+        // ---- one launch: tasks of one length that fit in memory together, as whole multiples of
+        //      the workgroups the device keeps resident (a partial extra wave would double the
+        //      launch), and balanced over the launches this length class needs ----
         const int lg = tasks[at].lg;
         size_t end = at, bytes = 0;
         while (end < tasks.size() && tasks[end].lg == lg && bytes + tasks[end].bytes <= budget) bytes += tasks[end++].bytes;
         if (end == at)
             return fail(e, CF_E_NOMEM, "one prime needs " + std::to_string(tasks[at].bytes) + " bytes of device memory, " +
                                            std::to_string(budget) + " usable");
+        size_t cls_end = end;
+        while (cls_end < tasks.size() && tasks[cls_end].lg == lg) ++cls_end;
+        const size_t fit = end - at, left = cls_end - at, cap = (size_t)std::max(resident_tasks(lg, e->device), 1);
+        size_t take = fit >= cap ? fit / cap * cap : fit;
+        if (take < left) {
+            const size_t launches = (left + take - 1) / take;
+            const size_t even = (left + launches - 1) / launches;
+            take = std::min(take, (even + cap - 1) / cap * cap);
+        }
+        end = at + std::min(take, left);
+        bytes = 0;
+        for (size_t k = at; k < end; ++k) bytes += tasks[k].bytes;
         const int nt = (int)(end - at);
         std::vector<TaskDesc> td(nt);
         std::vector<QueryDesc> qd;

@@ -87,6 +87,24 @@ __global__ void __launch_bounds__(Plan<LG>::T / kVirtual<LG>)
     for (int j = threadIdx.x; j < c.d->q_cnt; j += TP) res[qs[c.d->q_off + j].out] = (uint64_t)Task<LG>::result(j, c);
 }
 
+// Workgroups of the log2L kernel that the device runs at the same time (CUs x blocks per CU).
+int resident_tasks(int log2L, int device) {
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, device) != hipSuccess) return 0;
+    int per = 0;
+    hipError_t st = hipErrorInvalidValue;
+    switch (log2L) {
+        case 10: st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_tasks<10>, Plan<10>::T / kVirtual<10>, 0); break;
+        case 11: st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_tasks<11>, Plan<11>::T / kVirtual<11>, 0); break;
+        case 12: st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_tasks<12>, Plan<12>::T / kVirtual<12>, 0); break;
+        case 13: st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_tasks<13>, Plan<13>::T / kVirtual<13>, 0); break;
+        case 14: st = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_tasks<14>, Plan<14>::T / kVirtual<14>, 0); break;
+        default: break;
+    }
+    if (st != hipSuccess) return 0;
+    return pr.multiProcessorCount * (per > 0 ? per : 1);
+}
+
 hipError_t launch_tasks(int log2L, int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem,
                         uint64_t* res, int model, int dd, hipStream_t stream) {
     if (ntasks <= 0) return hipSuccess;
