@@ -181,6 +181,10 @@ struct TaskDesc {
     int pool;            // tiles in each of the two pools
     int NB, NS;          // 16-row blocks of the tables; staging rows (16 NB)
     int NT;              // threads of the task's workgroup (L / 16)
+    int lg;              // log2 L
+    int dd;              // d, the number of candidate slots per item
+    int model;           // 0: uniform d-subset, g(v) = C(v, d); 1: with replacement, g(v) = v^d
+    int pad_;
 };
 
 struct QueryDesc {

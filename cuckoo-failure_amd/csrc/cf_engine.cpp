@@ -75,10 +75,12 @@ int ensure(cf_engine* e, void** buf, size_t* cap, size_t bytes, const char* what
 }
 
 struct TaskPlan {
-    size_t b;                  // prime index in the call
+    uint64_t p;
+    int dd, model;
     int nm, F1, lg;
-    std::vector<int> queries;  // query indices, sorted by n descending
+    std::vector<QueryDesc> queries;  // sorted by n descending; out = index into the result array
     size_t bytes;
+    double cost;                     // estimated run time (arbitrary units), for ordering
 };
 
 // Free device memory that a batch may use: what is free now plus the arena we already hold,
@@ -89,6 +91,239 @@ size_t usable_bytes(cf_engine* e) {
     const size_t have = fr + e->arena_cap;
     const size_t reserve = (size_t)2 << 30;
     return have > reserve ? have - reserve : 0;
+}
+
+
+// One task: prime p, its queries (nq of them, query(k) giving each), checked and costed.
+template <class Q>
+int plan_task(cf_engine* e, uint64_t p, int d, int model, size_t nq, Q query, TaskPlan& tp) {
+    tp.p = p;
+    tp.dd = d;
+    tp.model = model;
+    tp.nm = 0;
+    int Mmax = 0;
+    for (size_t k = 0; k < nq; ++k) {
+        tp.queries.push_back(query(k));
+        tp.nm = std::max(tp.nm, tp.queries.back().n);
+        Mmax = std::max(Mmax, tp.queries.back().M);
+    }
+    if (p < 3 || !(p & 1) || p >= (uint64_t(1) << 50) || (uint64_t)Mmax >= p)
+        return fail(e, CF_E_ARGS, "prime " + std::to_string(p) + " is not an odd modulus in (M, 2^50)");
+    std::stable_sort(tp.queries.begin(), tp.queries.end(), [](const QueryDesc& a, const QueryDesc& c) { return a.n > c.n; });
+    tp.F1 = std::max(tp.nm, Mmax) + 1;
+    tp.lg = engine_log2_length(tp.nm, p);
+    if (tp.lg < 0)
+        return fail(e, CF_E_UNSUPPORTED,
+                    "p - 1 of " + std::to_string(p) + " has too few factors of two for n = " + std::to_string(tp.nm));
+    if (tp.lg > kMaxLog2L)
+        return fail(e, CF_E_UNSUPPORTED, "n = " + std::to_string(tp.nm) + " needs a transform longer than 2^" +
+                                             std::to_string(kMaxLog2L) + ", which this build does not run");
+    tp.bytes = task_bytes(plan_for(e, tp.nm), tp.F1, (int)tp.queries.size(), tp.lg);
+    // two convolutions of length L per row, plus the tile traffic
+    tp.cost = 2.0 * tp.nm * ((double)(1 << tp.lg) * (tp.lg + 1)) + 8.0 * tp.nm * (double)tp.nm;
+    return CF_OK;
+}
+
+// Descriptors of tasks[at, end) (arena offsets from arena0, consecutive) and their queries.
+void describe(cf_engine* e, const std::vector<TaskPlan>& tasks, size_t at, size_t end, bool shared_arena,
+              std::vector<TaskDesc>& td, std::vector<QueryDesc>& qd, std::vector<int>& slots) {
+    std::map<int, int> slot_off;
+    int64_t arena_off = 0;
+    td.assign(end - at, TaskDesc{});
+    qd.clear();
+    slots.clear();
+    for (size_t k = at; k < end; ++k) {
+        const TaskPlan& tp = tasks[k];
+        TaskDesc& t = td[k - at];
+        memset(&t, 0, sizeof t);
+        task_constants(tp.p, tp.lg, t);
+        const SlotPlan& sp = plan_for(e, tp.nm);
+        auto so = slot_off.find(tp.nm);
+        if (so == slot_off.end()) {
+            so = slot_off.emplace(tp.nm, (int)slots.size()).first;
+            slots.insert(slots.end(), sp.segs.begin(), sp.segs.end());
+        }
+        t.arena = shared_arena ? 0 : arena_off;
+        t.nm = tp.nm;
+        t.F1 = tp.F1;
+        t.q_off = (int)qd.size();
+        t.q_cnt = (int)tp.queries.size();
+        t.slot_off = so->second;
+        t.pool = sp.pool;
+        t.NB = sp.NB;
+        t.NSEG = sp.NSEG;
+        t.NS = sp.NS;
+        t.NT = (1 << tp.lg) / 16;
+        t.lg = tp.lg;
+        t.dd = tp.dd;
+        t.model = tp.model;
+        qd.insert(qd.end(), tp.queries.begin(), tp.queries.end());
+        arena_off += (int64_t)(tp.bytes / 8);
+    }
+}
+
+// Uploads descriptors, queries and slot tables (and a zero queue position) into the small
+// buffer; returns the device pointers.
+struct Uploaded {
+    const QueueJob* jobs;
+    const TaskDesc* td;
+    const QueryDesc* qd;
+    const int* slots;
+    uint64_t* res;
+    unsigned* next;
+};
+
+int upload(cf_engine* e, const std::vector<TaskDesc>& td, const std::vector<QueryDesc>& qd, const std::vector<int>& slots,
+           size_t nres, Uploaded& u, const std::vector<QueueJob>* jobs = nullptr) {
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t njobs = jobs ? jobs->size() : 0;
+    const size_t o_jb = 0, o_td = o_jb + up(sizeof(QueueJob) * njobs), o_qd = o_td + up(sizeof(TaskDesc) * td.size());
+    const size_t o_sl = o_qd + up(sizeof(QueryDesc) * qd.size());
+    const size_t o_nx = o_sl + up(sizeof(int) * slots.size());
+    const size_t o_rs = o_nx + 256;
+    const size_t small = o_rs + sizeof(uint64_t) * std::max<size_t>(nres, 1);
+    int rc = ensure(e, &e->small, &e->small_cap, small, "the task descriptors");
+    if (rc) return rc;
+    char* sm = (char*)e->small;
+    static const unsigned zero = 0;
+    if (njobs)
+        HIP_TRY(e, hipMemcpyAsync(sm + o_jb, jobs->data(), sizeof(QueueJob) * njobs, hipMemcpyHostToDevice, e->stream), "upload");
+    HIP_TRY(e, hipMemcpyAsync(sm + o_td, td.data(), sizeof(TaskDesc) * td.size(), hipMemcpyHostToDevice, e->stream), "upload");
+    HIP_TRY(e, hipMemcpyAsync(sm + o_qd, qd.data(), sizeof(QueryDesc) * qd.size(), hipMemcpyHostToDevice, e->stream), "upload");
+    HIP_TRY(e, hipMemcpyAsync(sm + o_sl, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, e->stream), "upload");
+    HIP_TRY(e, hipMemcpyAsync(sm + o_nx, &zero, sizeof zero, hipMemcpyHostToDevice, e->stream), "upload");
+    u.jobs = (const QueueJob*)(sm + o_jb);
+    u.td = (const TaskDesc*)(sm + o_td);
+    u.qd = (const QueryDesc*)(sm + o_qd);
+    u.slots = (const int*)(sm + o_sl);
+    u.next = (unsigned*)(sm + o_nx);
+    u.res = (uint64_t*)(sm + o_rs);
+    return CF_OK;
+}
+
+// Times one launch (HIP events on the engine's stream), then copies back the results of its
+// queries.
+template <class L>
+int timed_launch(cf_engine* e, L launch, const std::vector<QueryDesc>& qd, const Uploaded& u, size_t nres, uint64_t* res,
+                 float& ms) {
+    HIP_TRY(e, hipEventRecord(e->ev0, e->stream), "hipEventRecord");
+    HIP_TRY(e, launch(), "kernel launch");
+    HIP_TRY(e, hipEventRecord(e->ev1, e->stream), "hipEventRecord");
+    HIP_TRY(e, hipStreamSynchronize(e->stream), "kernel execution");
+    HIP_TRY(e, hipEventElapsedTime(&ms, e->ev0, e->ev1), "hipEventElapsedTime");
+    std::vector<uint64_t> out(nres);
+    HIP_TRY(e, hipMemcpy(out.data(), u.res, sizeof(uint64_t) * nres, hipMemcpyDeviceToHost), "download");
+    for (const QueryDesc& q : qd) res[q.out] = out[q.out];
+    return CF_OK;
+}
+
+// Runs every task. With queue, the tasks whose transform fits the work-queue kernel go to one
+// persistent launch (longest first); the rest, and everything without queue, run as launches
+// of one transform length each, packed into device memory.
+int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* res, bool queue) {
+    const size_t budget = usable_bytes(e);
+    float kernel_ms_total = 0;
+    std::vector<TaskDesc> td;
+    std::vector<QueryDesc> qd;
+    std::vector<int> slots;
+    if (queue) {
+        auto mid = std::stable_partition(tasks.begin(), tasks.end(), [](const TaskPlan& t) { return t.lg <= kQueueMaxLog2L; });
+        const size_t nq = (size_t)(mid - tasks.begin());
+        // longest first; tasks of one shape adjacent, so they can share a job
+        auto shape = [](const TaskPlan& t) {
+            std::vector<int> k = {t.lg, t.nm, t.dd, t.model, (int)t.queries.size()};
+            for (const QueryDesc& q : t.queries) k.push_back(q.n);
+            return k;
+        };
+        std::stable_sort(tasks.begin(), mid, [&](const TaskPlan& a, const TaskPlan& b) {
+            return a.cost != b.cost ? a.cost > b.cost : shape(a) < shape(b);
+        });
+        std::vector<QueueJob> jobs;
+        size_t stride = 0;
+        for (size_t k = 0; k < nq;) {
+            const int lg = tasks[k].lg, G = kQueueSlots(lg);
+            const std::vector<int> key = shape(tasks[k]);
+            size_t j = k + 1;
+            while (j < nq && (int)(j - k) < G && shape(tasks[j]) == key) ++j;
+            jobs.push_back(QueueJob{(int)k, (int)(j - k), lg, 0});
+            for (size_t i = k; i < j; ++i) stride = std::max(stride, (size_t)G * ((tasks[i].bytes + 255) & ~(size_t)255));
+            k = j;
+        }
+        if (nq) {
+            stride = (stride + 1023) & ~(size_t)1023;  // every slot stride a multiple of 256 bytes
+            const size_t wg = std::min({jobs.size(), (size_t)std::max(queue_resident(e->device), 1), budget / stride});
+            if (!wg)
+                return fail(e, CF_E_NOMEM, "one job needs " + std::to_string(stride) + " bytes of device memory, " +
+                                               std::to_string(budget) + " usable");
+            int rc = ensure(e, &e->arena, &e->arena_cap, wg * stride, "the task tables");
+            if (rc) return rc;
+            describe(e, tasks, 0, nq, true, td, qd, slots);
+            Uploaded u;
+            rc = upload(e, td, qd, slots, nres, u, &jobs);
+            if (rc) return rc;
+            float ms = 0;
+            rc = timed_launch(e, [&] {
+                return launch_queue((int)wg, (int)jobs.size(), u.jobs, u.td, u.qd, u.slots, (double*)e->arena,
+                                    (int64_t)(stride / 8), u.res, u.next, e->stream);
+            }, qd, u, nres, res, ms);
+            if (rc) return rc;
+            kernel_ms_total += ms;
+            e->stats.launches++;
+            e->stats.tasks += (int)nq;
+            e->stats.log2_length = std::max(e->stats.log2_length, tasks[0].lg);
+            e->stats.device_bytes = std::max(e->stats.device_bytes, wg * stride);
+        }
+        tasks.erase(tasks.begin(), mid);
+    }
+    // largest transforms first, then by table size, so a launch holds tasks of one length
+    std::stable_sort(tasks.begin(), tasks.end(), [](const TaskPlan& a, const TaskPlan& b) {
+        return a.lg != b.lg ? a.lg > b.lg : a.nm > b.nm;
+    });
+    size_t at = 0;
+    while (at < tasks.size()) {
+        // ---- one launch: tasks of one length that fit in memory together, as whole multiples of
+        //      the workgroups the device keeps resident (a partial extra wave would double the
+        //      launch), and balanced over the launches this length class needs ----
+        const int lg = tasks[at].lg;
+        size_t end = at, bytes = 0;
+        while (end < tasks.size() && tasks[end].lg == lg && bytes + tasks[end].bytes <= budget) bytes += tasks[end++].bytes;
+        if (end == at)
+            return fail(e, CF_E_NOMEM, "one prime needs " + std::to_string(tasks[at].bytes) + " bytes of device memory, " +
+                                           std::to_string(budget) + " usable");
+        size_t cls_end = end;
+        while (cls_end < tasks.size() && tasks[cls_end].lg == lg) ++cls_end;
+        const size_t fit = end - at, left = cls_end - at, cap = (size_t)std::max(resident_tasks(lg, e->device), 1);
+        size_t take = fit >= cap ? fit / cap * cap : fit;
+        if (take < left) {
+            const size_t launches = (left + take - 1) / take;
+            const size_t even = (left + launches - 1) / launches;
+            take = std::min(take, (even + cap - 1) / cap * cap);
+        }
+        end = at + std::min(take, left);
+        bytes = 0;
+        for (size_t k = at; k < end; ++k) bytes += tasks[k].bytes;
+        const int nt = (int)(end - at);
+        describe(e, tasks, at, end, false, td, qd, slots);
+        int rc = ensure(e, &e->arena, &e->arena_cap, bytes, "the task tables");
+        if (rc) return rc;
+        Uploaded u;
+        rc = upload(e, td, qd, slots, nres, u);
+        if (rc) return rc;
+        float ms = 0;
+        rc = timed_launch(e, [&] {
+            return launch_tasks(lg, nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, e->stream);
+        }, qd, u, nres, res, ms);
+        if (rc) return rc;
+        kernel_ms_total += ms;
+        e->stats.launches++;
+        e->stats.tasks += nt;
+        e->stats.log2_length = std::max(e->stats.log2_length, lg);
+        e->stats.device_bytes = std::max(e->stats.device_bytes, bytes);
+        at = end;
+    }
+    e->stats.batches = e->stats.launches;
+    e->stats.kernel_ms = kernel_ms_total;
+    return CF_OK;
 }
 
 }  // namespace
@@ -183,130 +418,57 @@ int cf_run_fused(cf_engine* e, int d, int model, const int32_t* n, const int32_t
     // ---- one task per prime that has any query ----
     std::vector<TaskPlan> tasks;
     for (size_t b = 0; b < np; ++b) {
-        const uint64_t p = primes[b];
-        TaskPlan tp;
-        tp.b = b;
-        tp.nm = 0;
-        int Mmax = 0;
+        std::vector<int> qi;
         for (size_t j = 0; j < J; ++j)
-            if (!mask || mask[b * J + j]) {
-                tp.queries.push_back((int)j);
-                tp.nm = std::max(tp.nm, (int)n[j]);
-                Mmax = std::max(Mmax, (int)M[j]);
-            }
-        if (tp.queries.empty()) continue;
-        if (p < 3 || !(p & 1) || p >= (uint64_t(1) << 50) || (uint64_t)Mmax >= p)
-            return fail(e, CF_E_ARGS, "prime " + std::to_string(p) + " is not an odd modulus in (M, 2^50)");
-        std::stable_sort(tp.queries.begin(), tp.queries.end(), [&](int a, int c) { return n[a] > n[c]; });
-        tp.F1 = std::max(tp.nm, Mmax) + 1;
-        tp.lg = engine_log2_length(tp.nm, p);
-        if (tp.lg < 0)
-            return fail(e, CF_E_UNSUPPORTED, "p - 1 of " + std::to_string(p) + " has too few factors of two for n = " +
-                                                 std::to_string(tp.nm));
-        if (tp.lg > kMaxLog2L)
-            return fail(e, CF_E_UNSUPPORTED, "n = " + std::to_string(tp.nm) + " needs a transform longer than 2^" +
-                                                 std::to_string(kMaxLog2L) + ", which this build does not run");
-        tp.bytes = task_bytes(plan_for(e, tp.nm), tp.F1, (int)tp.queries.size(), tp.lg);
+            if (!mask || mask[b * J + j]) qi.push_back((int)j);
+        if (qi.empty()) continue;
+        TaskPlan tp;
+        int rc = plan_task(e, primes[b], d, model, qi.size(), [&](size_t k) {
+            const int j = qi[k];
+            return QueryDesc{n[j], M[j], (int64_t)(b * J + j)};
+        }, tp);
+        if (rc) return rc;
         tasks.push_back(std::move(tp));
     }
-    // largest transforms first, then by table size, so a launch holds tasks of one length
-    std::stable_sort(tasks.begin(), tasks.end(), [](const TaskPlan& a, const TaskPlan& b) {
-        return a.lg != b.lg ? a.lg > b.lg : a.nm > b.nm;
-    });
-
-    const size_t budget = usable_bytes(e);
-    size_t at = 0;
-    float kernel_ms_total = 0;
-    while (at < tasks.size()) {
-        // ---- one launch: tasks of one length that fit in memory together, as whole multiples of
-        //      the workgroups the device keeps resident (a partial extra wave would double the
-        //      launch), and balanced over the launches this length class needs ----
-        const int lg = tasks[at].lg;
-        size_t end = at, bytes = 0;
-        while (end < tasks.size() && tasks[end].lg == lg && bytes + tasks[end].bytes <= budget) bytes += tasks[end++].bytes;
-        if (end == at)
-            return fail(e, CF_E_NOMEM, "one prime needs " + std::to_string(tasks[at].bytes) + " bytes of device memory, " +
-                                           std::to_string(budget) + " usable");
-        size_t cls_end = end;
-        while (cls_end < tasks.size() && tasks[cls_end].lg == lg) ++cls_end;
-        const size_t fit = end - at, left = cls_end - at, cap = (size_t)std::max(resident_tasks(lg, e->device), 1);
-        size_t take = fit >= cap ? fit / cap * cap : fit;
-        if (take < left) {
-            const size_t launches = (left + take - 1) / take;
-            const size_t even = (left + launches - 1) / launches;
-            take = std::min(take, (even + cap - 1) / cap * cap);
-        }
-        end = at + std::min(take, left);
-        bytes = 0;
-        for (size_t k = at; k < end; ++k) bytes += tasks[k].bytes;
-        const int nt = (int)(end - at);
-        std::vector<TaskDesc> td(nt);
-        std::vector<QueryDesc> qd;
-        std::vector<int> slots;
-        std::map<int, int> slot_off;
-        int64_t arena_off = 0;
-        for (int k = 0; k < nt; ++k) {
-            const TaskPlan& tp = tasks[at + k];
-            TaskDesc& t = td[k];
-            memset(&t, 0, sizeof t);
-            task_constants(primes[tp.b], lg, t);
-            const SlotPlan& sp = plan_for(e, tp.nm);
-            auto so = slot_off.find(tp.nm);
-            if (so == slot_off.end()) {
-                so = slot_off.emplace(tp.nm, (int)slots.size()).first;
-                slots.insert(slots.end(), sp.segs.begin(), sp.segs.end());
-            }
-            t.arena = arena_off;
-            t.nm = tp.nm;
-            t.F1 = tp.F1;
-            t.q_off = (int)qd.size();
-            t.q_cnt = (int)tp.queries.size();
-            t.slot_off = so->second;
-            t.pool = sp.pool;
-            t.NB = sp.NB;
-            t.NSEG = sp.NSEG;
-            t.NS = sp.NS;
-            t.NT = (1 << lg) / 16;
-            for (int j : tp.queries) qd.push_back(QueryDesc{n[j], M[j], (int64_t)(tp.b * J + j)});
-            arena_off += (int64_t)(tp.bytes / 8);
-        }
-        // ---- device buffers ----
-        int rc = ensure(e, &e->arena, &e->arena_cap, bytes, "the task tables");
-        if (rc) return rc;
-        const size_t o_td = 0, o_qd = o_td + ((sizeof(TaskDesc) * nt + 255) & ~(size_t)255);
-        const size_t o_sl = o_qd + ((sizeof(QueryDesc) * qd.size() + 255) & ~(size_t)255);
-        const size_t o_rs = o_sl + ((sizeof(int) * slots.size() + 255) & ~(size_t)255);
-        const size_t small = o_rs + sizeof(uint64_t) * std::max<size_t>(np * J, 1);
-        rc = ensure(e, &e->small, &e->small_cap, small, "the task descriptors");
-        if (rc) return rc;
-        char* sm = (char*)e->small;
-        HIP_TRY(e, hipMemcpyAsync(sm + o_td, td.data(), sizeof(TaskDesc) * nt, hipMemcpyHostToDevice, e->stream), "upload");
-        HIP_TRY(e, hipMemcpyAsync(sm + o_qd, qd.data(), sizeof(QueryDesc) * qd.size(), hipMemcpyHostToDevice, e->stream), "upload");
-        HIP_TRY(e, hipMemcpyAsync(sm + o_sl, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, e->stream), "upload");
-        // ---- launch ----
-        HIP_TRY(e, hipEventRecord(e->ev0, e->stream), "hipEventRecord");
-        HIP_TRY(e, launch_tasks(lg, nt, (const TaskDesc*)(sm + o_td), (const QueryDesc*)(sm + o_qd), (const int*)(sm + o_sl),
-                                (double*)e->arena, (uint64_t*)(sm + o_rs), model, d, e->stream),
-                "kernel launch");
-        HIP_TRY(e, hipEventRecord(e->ev1, e->stream), "hipEventRecord");
-        HIP_TRY(e, hipStreamSynchronize(e->stream), "kernel execution");
-        float ms = 0;
-        HIP_TRY(e, hipEventElapsedTime(&ms, e->ev0, e->ev1), "hipEventElapsedTime");
-        kernel_ms_total += ms;
-        // ---- results: the kernel wrote only the masked-in entries of this launch ----
-        std::vector<uint64_t> out(np * J);
-        HIP_TRY(e, hipMemcpy(out.data(), sm + o_rs, sizeof(uint64_t) * np * J, hipMemcpyDeviceToHost), "download");
-        for (const QueryDesc& q : qd) res[q.out] = out[q.out];
-        e->stats.launches++;
-        e->stats.tasks += nt;
-        e->stats.log2_length = lg;
-        e->stats.device_bytes = std::max(e->stats.device_bytes, bytes);
-        at = end;
-    }
-    e->stats.batches = e->stats.launches;
-    e->stats.kernel_ms = kernel_ms_total;
+    const int rc = execute(e, tasks, np * J, res, false);
     e->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    return CF_OK;
+    return rc;
+}
+
+int cf_run_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t* primes, uint64_t* res) {
+    if (!e) return CF_E_ARGS;
+    const auto t_start = std::chrono::steady_clock::now();
+    e->err.clear();
+    e->stats = cf_stats{};
+    if (k && (!inst || !primes || !res)) return fail(e, CF_E_ARGS, "bad arguments");
+    // the instances' prime ranges (and so their result ranges) must not overlap
+    std::vector<std::pair<int64_t, int64_t>> rng;
+    int64_t nres = 0;
+    for (size_t i = 0; i < k; ++i) {
+        const cf_instance& x = inst[i];
+        if (x.d < 1 || (x.model != 0 && x.model != 1) || x.n < 1 || x.M <= x.n || x.prime_off < 0 || x.prime_cnt < 0)
+            return fail(e, CF_E_ARGS, "bad instance " + std::to_string(i));
+        if (x.prime_cnt) rng.emplace_back(x.prime_off, x.prime_off + x.prime_cnt);
+        nres = std::max(nres, x.prime_off + x.prime_cnt);
+    }
+    std::sort(rng.begin(), rng.end());
+    for (size_t i = 1; i < rng.size(); ++i)
+        if (rng[i].first < rng[i - 1].second) return fail(e, CF_E_ARGS, "instances share prime/result entries");
+    for (const auto& r : rng) std::fill(res + r.first, res + r.second, 0);
+    HIP_TRY(e, hipSetDevice(e->device), "hipSetDevice");
+    std::vector<TaskPlan> tasks;
+    for (size_t i = 0; i < k; ++i) {
+        const cf_instance& x = inst[i];
+        for (int64_t b = x.prime_off; b < x.prime_off + x.prime_cnt; ++b) {
+            TaskPlan tp;
+            int rc = plan_task(e, primes[b], x.d, x.model, 1, [&](size_t) { return QueryDesc{x.n, x.M, b}; }, tp);
+            if (rc) return rc;
+            tasks.push_back(std::move(tp));
+        }
+    }
+    const int rc = execute(e, tasks, (size_t)nres, res, true);
+    e->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return rc;
 }
 
 // ---- host logic, no device ----

@@ -62,29 +62,104 @@ struct DevRunner {
     }
 };
 
+// The task's context from its descriptor; arena_base is where its arena offset counts from.
+__device__ __forceinline__ Ctx make_ctx(const TaskDesc* d, const QueryDesc* qs, const int* slots, double* arena_base) {
+    Ctx c;
+    c.d = d;
+    c.m.p = d->p;
+    c.m.pinv = d->pinv;
+    c.a = arena_of(arena_base, *d);
+    c.slots = slots + d->slot_off;
+    c.q = qs;
+    c.model = d->model;
+    c.dd = d->dd;
+    c.v0 = d->model ? 1 : d->dd;
+    return c;
+}
+
 template <int LG>
 constexpr int kVirtual = LG >= 14 ? 2 : 1;
 
 template <int LG>
 __global__ void __launch_bounds__(Plan<LG>::T / kVirtual<LG>)
     k_tasks(const TaskDesc* __restrict__ tasks, const QueryDesc* __restrict__ qs, const int* __restrict__ slots,
-            double* __restrict__ mem, uint64_t* __restrict__ res, int model, int dd) {
+            double* __restrict__ mem, uint64_t* __restrict__ res) {
     constexpr int VT = kVirtual<LG>, TP = Plan<LG>::T / VT;
     __shared__ Shared<LG> sh;
     DevRunner<VT, TP> run;
     run.t = (int)threadIdx.x;
-    Ctx c;
-    c.d = tasks + blockIdx.x;
-    c.m.p = c.d->p;
-    c.m.pinv = c.d->pinv;
-    c.a = arena_of(mem, *c.d);
-    c.slots = slots + c.d->slot_off;
-    c.q = qs;
-    c.model = model;
-    c.dd = dd;
-    c.v0 = model ? 1 : dd;
+    const Ctx c = make_ctx(tasks + blockIdx.x, qs, slots, mem);
     Task<LG>::run_task(run, sh, c);
     for (int j = threadIdx.x; j < c.d->q_cnt; j += TP) res[qs[c.d->q_off + j].out] = (uint64_t)Task<LG>::result(j, c);
+}
+
+// ---- the work-queue kernel: a persistent grid over jobs of transform lengths 2^10 .. 2^12 ----
+// A job is 1, 2 or 4 tasks (L = 2^12, 2^11, 2^10) that the workgroup runs side by side, each on
+// its own T = L/16 threads (whole waves), LDS slice and arena slice. The host puts only tasks of
+// identical shape (nm, d, model, query sizes) into one job, so their phase sequences are the same
+// and they share the workgroup barriers; a job with fewer tasks than slots repeats its first
+// task in the spare slots and discards those results. Workgroups take jobs from the queue in
+// the host's order (longest first) until it is empty.
+constexpr int kQueueThreads = Plan<kQueueMaxLog2L>::T;
+
+union QueueLds {
+    Shared<12> s12;
+    Shared<11> s11[2];
+    Shared<10> s10[4];
+};
+
+template <int LG>
+__device__ __forceinline__ void queue_job(QueueLds& lds, const QueueJob& jb, const TaskDesc* tasks, const QueryDesc* qs,
+                                          const int* slots, double* arena, int64_t slot_stride, uint64_t* res) {
+    constexpr int T = Plan<LG>::T, G = kQueueThreads / T;
+    static_assert(G * sizeof(Shared<LG>) <= sizeof(QueueLds), "LDS");
+    const int g = (int)threadIdx.x / T;  // uniform over each wave, since T is a multiple of 64
+    Shared<LG>& sh = reinterpret_cast<Shared<LG>*>(&lds)[g];
+    const int k = __builtin_amdgcn_readfirstlane(jb.first + (g < jb.cnt ? g : 0));
+    DevRunner<1, T> run;
+    run.t = (int)threadIdx.x - g * T;
+    const Ctx c = make_ctx(tasks + k, qs, slots, arena + slot_stride * g);
+    Task<LG>::run_task(run, sh, c);
+    if (g < jb.cnt)
+        for (int j = run.t; j < c.d->q_cnt; j += T) res[qs[c.d->q_off + j].out] = (uint64_t)Task<LG>::result(j, c);
+}
+
+__global__ void __launch_bounds__(kQueueThreads, 2)
+    k_queue(const QueueJob* __restrict__ jobs, int njobs, const TaskDesc* __restrict__ tasks, const QueryDesc* __restrict__ qs,
+            const int* __restrict__ slots, double* __restrict__ mem, int64_t arena_stride, uint64_t* __restrict__ res,
+            unsigned* __restrict__ next) {
+    __shared__ QueueLds lds;
+    __shared__ int take;
+    double* arena = mem + arena_stride * blockIdx.x;
+    for (;;) {
+        if (threadIdx.x == 0) take = (int)atomicAdd(next, 1u);
+        __syncthreads();
+        const int k = take;
+        __syncthreads();  // every thread has read take before thread 0 writes the next one
+        if (k >= njobs) break;
+        const QueueJob jb = jobs[k];
+        switch (jb.lg) {
+            case 10: queue_job<10>(lds, jb, tasks, qs, slots, arena, arena_stride / 4, res); break;
+            case 11: queue_job<11>(lds, jb, tasks, qs, slots, arena, arena_stride / 2, res); break;
+            default: queue_job<12>(lds, jb, tasks, qs, slots, arena, arena_stride, res); break;
+        }
+        __syncthreads();  // the results are written before the next job reuses the arena
+    }
+}
+
+int queue_resident(int device) {
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, device) != hipSuccess) return 0;
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_queue, kQueueThreads, 0) != hipSuccess) return 0;
+    return pr.multiProcessorCount * (per > 0 ? per : 1);
+}
+
+hipError_t launch_queue(int workgroups, int njobs, const QueueJob* jobs, const TaskDesc* tasks, const QueryDesc* qs,
+                        const int* slots, double* mem, int64_t arena_stride, uint64_t* res, unsigned* next, hipStream_t stream) {
+    if (njobs <= 0 || workgroups <= 0) return hipSuccess;
+    k_queue<<<dim3((unsigned)workgroups), kQueueThreads, 0, stream>>>(jobs, njobs, tasks, qs, slots, mem, arena_stride, res, next);
+    return hipGetLastError();
 }
 
 // Workgroups of the log2L kernel that the device runs at the same time (CUs x blocks per CU).
@@ -106,15 +181,15 @@ int resident_tasks(int log2L, int device) {
 }
 
 hipError_t launch_tasks(int log2L, int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem,
-                        uint64_t* res, int model, int dd, hipStream_t stream) {
+                        uint64_t* res, hipStream_t stream) {
     if (ntasks <= 0) return hipSuccess;
     const dim3 grid((unsigned)ntasks);
     switch (log2L) {
-        case 10: k_tasks<10><<<grid, Plan<10>::T / kVirtual<10>, 0, stream>>>(tasks, qs, slots, mem, res, model, dd); break;
-        case 11: k_tasks<11><<<grid, Plan<11>::T / kVirtual<11>, 0, stream>>>(tasks, qs, slots, mem, res, model, dd); break;
-        case 12: k_tasks<12><<<grid, Plan<12>::T / kVirtual<12>, 0, stream>>>(tasks, qs, slots, mem, res, model, dd); break;
-        case 13: k_tasks<13><<<grid, Plan<13>::T / kVirtual<13>, 0, stream>>>(tasks, qs, slots, mem, res, model, dd); break;
-        case 14: k_tasks<14><<<grid, Plan<14>::T / kVirtual<14>, 0, stream>>>(tasks, qs, slots, mem, res, model, dd); break;
+        case 10: k_tasks<10><<<grid, Plan<10>::T / kVirtual<10>, 0, stream>>>(tasks, qs, slots, mem, res); break;
+        case 11: k_tasks<11><<<grid, Plan<11>::T / kVirtual<11>, 0, stream>>>(tasks, qs, slots, mem, res); break;
+        case 12: k_tasks<12><<<grid, Plan<12>::T / kVirtual<12>, 0, stream>>>(tasks, qs, slots, mem, res); break;
+        case 13: k_tasks<13><<<grid, Plan<13>::T / kVirtual<13>, 0, stream>>>(tasks, qs, slots, mem, res); break;
+        case 14: k_tasks<14><<<grid, Plan<14>::T / kVirtual<14>, 0, stream>>>(tasks, qs, slots, mem, res); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -13,6 +13,20 @@ int resident_tasks(int log2L, int device);
 
 // One workgroup per task; log2L selects the kernel instance (transform length 2^log2L).
 hipError_t launch_tasks(int log2L, int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem,
-                        uint64_t* res, int model, int dd, hipStream_t stream);
+                        uint64_t* res, hipStream_t stream);
+
+// The work-queue kernel: jobs of tasks with log2 L in [kMinLog2L, kQueueMaxLog2L], any mix,
+// ordered as they should start. A job is up to kQueueSlots(lg) consecutive tasks of identical
+// shape that one workgroup runs side by side. `workgroups` persistent workgroups; workgroup w
+// uses the arena at mem + w * arena_stride (doubles), slot g of an lg job its part
+// g * arena_stride / kQueueSlots(lg). *next must be 0 at launch (the queue position).
+constexpr int kQueueMaxLog2L = 12;
+constexpr int kQueueSlots(int lg) { return 1 << (kQueueMaxLog2L - lg); }
+struct QueueJob {
+    int first, cnt, lg, pad_;
+};
+int queue_resident(int device);
+hipError_t launch_queue(int workgroups, int njobs, const QueueJob* jobs, const TaskDesc* tasks, const QueryDesc* qs,
+                        const int* slots, double* mem, int64_t arena_stride, uint64_t* res, unsigned* next, hipStream_t stream);
 
 }  // namespace cf

@@ -36,6 +36,11 @@ class Stats(ctypes.Structure):
                 ("device_bytes", ctypes.c_size_t)]
 
 
+class Instance(ctypes.Structure):
+    _fields_ = [("d", ctypes.c_int32), ("model", ctypes.c_int32), ("n", ctypes.c_int32), ("M", ctypes.c_int32),
+                ("prime_off", ctypes.c_int64), ("prime_cnt", ctypes.c_int64)]
+
+
 def lib():
     """The loaded engine library; raises if it has not been built."""
     global _lib
@@ -55,6 +60,8 @@ def lib():
         L.cf_run_fused.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
                                    ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t, ctypes.POINTER(u64), ctypes.c_size_t,
                                    ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(u64)]
+        L.cf_run_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Instance), ctypes.c_size_t, ctypes.POINTER(u64),
+                                   ctypes.POINTER(u64)]
         L.cf_max_batch.restype = ctypes.c_long
         L.cf_max_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
         L.cf_mem_per_prime.restype = ctypes.c_size_t
@@ -146,6 +153,22 @@ class Engine:
         if rc != CF_OK:
             raise CfError(rc, lib().cf_last_error(self._h).decode())
         return [[res[b * J + j] for j in range(J)] for b in range(np_)]
+
+    def run_batch(self, instances):
+        """Uneven batch in one work-queue launch. instances: (d, model, n, M, primes) tuples;
+        returns, per instance, N_bad(n, M) mod each of its primes."""
+        k = len(instances)
+        inst = (Instance * max(k, 1))()
+        flat = []
+        for i, (d, model, n, M, primes) in enumerate(instances):
+            inst[i] = Instance(d, model, n, M, len(flat), len(primes))
+            flat.extend(primes)
+        P = (u64 * max(len(flat), 1))(*flat)
+        res = (u64 * max(len(flat), 1))()
+        rc = lib().cf_run_batch(self._h, inst, k, P, res)
+        if rc != CF_OK:
+            raise CfError(rc, lib().cf_last_error(self._h).decode())
+        return [[res[inst[i].prime_off + b] for b in range(inst[i].prime_cnt)] for i in range(k)]
 
     def stats(self):
         s = Stats()

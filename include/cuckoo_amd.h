@@ -40,7 +40,7 @@ typedef struct {
     int launches;
     int tasks;
     int batches;
-    int log2_length; /* transform length of the last launch */
+    int log2_length; /* log2 of the longest transform of the call */
     size_t device_bytes;
 } cf_stats;
 
@@ -71,6 +71,23 @@ int cf_device_name(const cf_engine* e, char* buf, size_t len);
  * code; cf_last_error() has the detail. */
 int cf_run_fused(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, size_t J,
                  const uint64_t* primes, size_t np, const uint8_t* mask, uint64_t* res);
+
+/* One instance of an uneven batch: the query N_bad(n, M) of a d-choice table, modulo each of
+ * primes[prime_off .. prime_off + prime_cnt), with results at the same indices of res. */
+typedef struct {
+    int32_t d;      /* candidate slots per item (d >= 1) */
+    int32_t model;  /* 0: uniform d-subset; 1: d draws with replacement */
+    int32_t n, M;   /* 1 <= n < M */
+    int64_t prime_off, prime_cnt;
+} cf_instance;
+
+/* Uneven batch: every (instance, prime) table of k instances of different n, M and d in one
+ * persistent work-queue launch (transforms up to 2^12, i.e. n <= 4096; longer ones in launches
+ * of their own), longest first. The reference runs such a batch as one cuckoo_fast process per
+ * instance (fast/cuckoo_fast.cu:653-898 with D, QUERIES = "n M", FIRST = 0, COUNT = need); the
+ * residues are the same, res[i] = N_bad mod primes[i]. Instances must not share prime/result
+ * indices; indices no instance covers are left untouched. */
+int cf_run_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t* primes, uint64_t* res);
 
 /* How many primes of a table with nmax rows and factorials up to Mmax, answering J queries, fit
  * in the device memory now free (the reference's batch sizing, fast/cuckoo_fast.cu:742-750). */

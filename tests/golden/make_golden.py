@@ -7,6 +7,7 @@ the programs' outputs are kept, as data.
 
     python tests/golden/make_golden.py            # everything except the n = 65536 prime
     python tests/golden/make_golden.py --big      # also n = 65536 (about 32 GB RAM, ~3 min)
+    python tests/golden/make_golden.py --only uneven   # just the uneven-batch fixture
 """
 import json
 import os
@@ -158,8 +159,44 @@ def fail_cases():
     return recs
 
 
+def uneven_golden(n_max=512):
+    """The uneven-batch instance list (seed 0) and, for its instances with n <= n_max, the
+    reference's residues over the instance's whole prime set (one cuckoo_fast run per instance,
+    FIRST = 0, COUNT = need): the first three lines, and the CRT-combined exact N_bad as its
+    bit length and the SHA-256 of its little-endian bytes."""
+    import hashlib
+    sys.path.insert(0, os.path.join(REPO, "cuckoo-failure_amd"))
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    import workloads
+
+    inst = workloads.uneven_batch(0, 256)
+    small = [(i, n, d, M) for i, (n, d, M) in enumerate(inst) if n <= n_max]
+
+    def one(a):
+        i, n, d, M = a
+        k = oracle.need(n, M, d)
+        rec = fast_run(d, [(n, M)], 0, k)
+        assert rec["rc"] == 0, rec["stderr"]
+        lines = rec["stdout"].splitlines()
+        assert len(lines) == k
+        pairs = [(int(t.split()[1]), int(t.split()[4])) for t in lines]
+        N, _ = oracle.crt(pairs)
+        raw = N.to_bytes((N.bit_length() + 7) // 8, "little")
+        return {"index": i, "n": n, "d": d, "M": M, "need": k, "head": lines[:3],
+                "nbad_bits": N.bit_length(), "nbad_sha256": hashlib.sha256(raw).hexdigest()}
+
+    with ThreadPoolExecutor(max_workers=os.cpu_count()) as ex:
+        gold = list(ex.map(one, small))
+    return {"seed": 0, "instances": [list(x) for x in inst], "golden": gold}
+
+
 def main():
     os.makedirs(HERE, exist_ok=True)
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "uneven":
+        with open(os.path.join(HERE, "uneven_batch.json"), "w") as f:
+            json.dump(uneven_golden(), f, indent=0)
+        return
     rc, so, _ = run([CHECK])
     assert rc == 0, "reference check program failed"
     with open(os.path.join(HERE, "check_ref.txt"), "w") as f:
@@ -174,6 +211,8 @@ def main():
         json.dump(fail_cases(), f, indent=0)
     with open(os.path.join(HERE, "fast_16384.json"), "w") as f:
         json.dump(fast_big(16384, 32768, 0, 8), f, indent=0)
+    with open(os.path.join(HERE, "uneven_batch.json"), "w") as f:
+        json.dump(uneven_golden(), f, indent=0)
     if "--big" in sys.argv:
         with open(os.path.join(HERE, "fast_65536.json"), "w") as f:
             json.dump(fast_big(65536, 131072, 0, 1), f, indent=0)

@@ -192,3 +192,15 @@ def test_known_answer_readme():
     out = fail_case(["single", "-n", "20", "-M", "40", "-d", "3"])["stdout"]
     assert "primes: 10 (product 2^311.3)" in out
     assert "p_fail = 2.328850077437917e-07" in out
+
+
+def test_uneven_batch_instances_pinned():
+    """The committed uneven-batch instance list is what the seeded generator draws."""
+    import workloads
+    from golden_data import load
+
+    g = load("uneven_batch.json")
+    assert [tuple(x) for x in g["instances"]] == workloads.uneven_batch(g["seed"], 256)
+    for x in g["golden"]:
+        assert tuple(g["instances"][x["index"]]) == (x["n"], x["d"], x["M"])
+        assert oracle.need(x["n"], x["M"], x["d"]) == x["need"]
