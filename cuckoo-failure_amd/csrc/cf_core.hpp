@@ -185,7 +185,14 @@ struct TaskDesc {
     int dd;              // d, the number of candidate slots per item
     int model;           // 0: uniform d-subset, g(v) = C(v, d); 1: with replacement, g(v) = v^d
     int pad_;
+    double* keep;        // or null: resident table store of this prime (factorials, then the rows
+                         // a_v[w] = V[v][w] / (w! v! (v-1-w)!) of v = v0+1 .. nm, packed by row)
 };
+
+// The resident store of one prime: fact, ifact, inv (F1 each), then row v (v entries) at
+// 3 F1 + v (v - 1) / 2 for v = 1 .. nm (rows v <= v0 are zero and never written).
+CF_HD int64_t keep_doubles(int F1, int nm) { return 3 * (int64_t)F1 + (int64_t)nm * (nm + 1) / 2; }
+CF_HD int64_t keep_row(int F1, int v) { return 3 * (int64_t)F1 + (int64_t)v * (v - 1) / 2; }
 
 struct QueryDesc {
     int n, M;

@@ -18,24 +18,11 @@
 #include "../../include/cuckoo_amd.h"
 #include "cf_host.hpp"
 #include "cf_kernels.hpp"
+#include "cf_engine.hpp"
 
 using namespace cf;
 
-struct cf_engine {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::string err;
-    cf_stats stats{};
-    // device buffers, grown on demand and kept between calls
-    void* arena = nullptr;
-    size_t arena_cap = 0;
-    void* small = nullptr;  // tasks, queries, slot tables, results
-    size_t small_cap = 0;
-    std::map<int, SlotPlan> plans;  // by nm
-};
-
-namespace {
+namespace cf_int {
 
 int fail(cf_engine* e, int code, const std::string& msg) {
     if (e) e->err = msg;
@@ -45,12 +32,6 @@ int fail(cf_engine* e, int code, const std::string& msg) {
 int hip_fail(cf_engine* e, hipError_t st, const char* what) {
     return fail(e, CF_E_DEVICE, std::string("HIP error in ") + what + ": " + hipGetErrorString(st));
 }
-
-#define HIP_TRY(e, x, what)                              \
-    do {                                                 \
-        hipError_t st_ = (x);                            \
-        if (st_ != hipSuccess) return hip_fail(e, st_, what); \
-    } while (0)
 
 const SlotPlan& plan_for(cf_engine* e, int nm) {
     auto it = e->plans.find(nm);
@@ -73,15 +54,6 @@ int ensure(cf_engine* e, void** buf, size_t* cap, size_t bytes, const char* what
     *cap = bytes;
     return CF_OK;
 }
-
-struct TaskPlan {
-    uint64_t p;
-    int dd, model;
-    int nm, F1, lg;
-    std::vector<QueryDesc> queries;  // sorted by n descending; out = index into the result array
-    size_t bytes;
-    double cost;                     // estimated run time (arbitrary units), for ordering
-};
 
 // Free device memory that a batch may use: what is free now plus the arena we already hold,
 // minus a reserve for the small buffers and the runtime.
@@ -157,6 +129,7 @@ void describe(cf_engine* e, const std::vector<TaskPlan>& tasks, size_t at, size_
         t.lg = tp.lg;
         t.dd = tp.dd;
         t.model = tp.model;
+        t.keep = tp.keep;
         qd.insert(qd.end(), tp.queries.begin(), tp.queries.end());
         arena_off += (int64_t)(tp.bytes / 8);
     }
@@ -326,7 +299,9 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
     return CF_OK;
 }
 
-}  // namespace
+}  // namespace cf_int
+
+using namespace cf_int;
 
 extern "C" {
 

@@ -1,0 +1,59 @@
+// cf_engine.hpp -- the engine's internals shared by the C ABI's source files: the engine
+// object, task plans and the executor that runs them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/cuckoo_amd.h"
+#include "cf_core.hpp"
+#include "cf_host.hpp"
+
+struct cf_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    cf_stats stats{};
+    // device buffers, grown on demand and kept between calls
+    void* arena = nullptr;
+    size_t arena_cap = 0;
+    void* small = nullptr;  // tasks, queries, slot tables, results
+    size_t small_cap = 0;
+    std::map<int, cf::SlotPlan> plans;  // by nm
+};
+
+namespace cf_int {
+
+using cf::QueryDesc;
+
+struct TaskPlan {
+    uint64_t p;
+    int dd, model;
+    int nm, F1, lg;
+    std::vector<QueryDesc> queries;  // sorted by n descending; out = index into the result array
+    size_t bytes;
+    double cost;                     // estimated run time (arbitrary units), for ordering
+    double* keep = nullptr;          // retain mode: the prime's resident table store
+};
+
+int fail(cf_engine* e, int code, const std::string& msg);
+int hip_fail(cf_engine* e, hipError_t st, const char* what);
+
+#define HIP_TRY(e, x, what)                                   \
+    do {                                                      \
+        hipError_t st_ = (x);                                 \
+        if (st_ != hipSuccess) return cf_int::hip_fail(e, st_, what); \
+    } while (0)
+
+const cf::SlotPlan& plan_for(cf_engine* e, int nm);
+// Grows *buf (device memory) to at least bytes.
+int ensure(cf_engine* e, void** buf, size_t* cap, size_t bytes, const char* what);
+// Device memory a call may use: free now plus the engine's arena, minus a reserve.
+size_t usable_bytes(cf_engine* e);
+// Runs every task (see cf_engine.cpp); res gets each query's residue at its out index.
+int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* res, bool queue);
+
+}  // namespace cf_int

@@ -1,0 +1,40 @@
+// cf_serve.hpp -- launcher of the query kernels on resident tables (cf_serve.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cf {
+
+constexpr int kServeThreads = 512;  // 8 waves per workgroup, one row range each
+constexpr int kServeJ = 32;         // queries per pass (one accumulator each per lane)
+
+struct ServePrime {
+    double p, pinv;
+};
+
+struct ServeQuery {
+    int n, M;
+    int need;  // primes 0 .. need-1 of the table set answer this query
+    int pad_;
+};
+
+struct ServeArgs {
+    const double* store;     // resident stores, stride doubles apart
+    int64_t stride;
+    const ServePrime* primes;
+    int np, F1;              // primes; factorial tables 0 .. F1-1
+    int dd, model, v0;       // the table set's d and model; rows v <= v0 are zero
+    const ServeQuery* q;     // the batch
+    int J;
+    const int* bounds;       // per pass z: row ranges (bounds[z (ranges+1) + r], ... + r + 1]
+    int ranges;              // ranges per pass: groups x the waves per workgroup
+    int passes, groups;      // passes = ceil(J / kServeJ)
+    double* part;            // np x ranges x J partial sums
+};
+
+// res[b J + j] = N_bad(n_j, M_j) mod p_b for b < need_j, else 0. The queries must be sorted by
+// n, largest first.
+hipError_t launch_serve(const ServeArgs& a, uint64_t* res, hipStream_t stream);
+
+}  // namespace cf

@@ -339,6 +339,27 @@ struct Task {
         }
     }
 
+    // Retain mode: row v of the stage-1 input (a[w] for w < v, in th.x) into the resident store.
+    static CF_HD void keep_row_out(const Thread& th, int t, int v, const Ctx& c) {
+        double* keep = c.d->keep;
+        if (!keep) return;
+        double* row = keep + keep_row(c.d->F1, v);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = k * T + t;
+            if (e < v) row[e] = th.x[k];
+        }
+    }
+    static CF_HD void keep_factorials(int t, const Ctx& c) {
+        double* keep = c.d->keep;
+        const int F1 = c.d->F1;
+        for (int i = t; i < F1; i += T) {
+            keep[i] = c.a.fact[i];
+            keep[F1 + i] = c.a.ifact[i];
+            keep[2 * F1 + i] = c.a.inv[i];
+        }
+    }
+
     // Fetch ahead for step v + 1: its row of P2 except column v (which the hand-over carries).
     static CF_HD void fetch_next(Thread& th, const Sh& sh, int t, int v, const Ctx& c) {
         fetch_row(th.pa, c.a.Vpool, c.a.Vst, sh, t, v + 1, 0, v, c);
@@ -391,6 +412,7 @@ struct Task {
                     // stage-1 input, its query terms and the first forward round in one phase
                     run.phase([&](Thread& th, int t) {
                         stage1_pre(th, sh, t, v, now, c);
+                        keep_row_out(th, t, v, c);
                         if (cnt0) run.wave_partial8(sh.red, th, t, [&](Thread& th2, int t2, double (&acc)[8]) {
                             query_partial(th2, t2, v, 0, cnt0, acc, c);
                             return cnt0;
@@ -398,7 +420,10 @@ struct Task {
                         E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
                     });
                 } else {
-                    run.phase([&](Thread& th, int t) { stage1_pre(th, sh, t, v, now, c); });
+                    run.phase([&](Thread& th, int t) {
+                        stage1_pre(th, sh, t, v, now, c);
+                        keep_row_out(th, t, v, c);
+                    });
                     queries(run, sh, v, 0, c);
                     run.phase([&](Thread& th, int t) { E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0); });
                 }
@@ -461,9 +486,13 @@ struct Task {
         if (nm > c.v0) {
             if ((nm >> 4) != segY) run.phase([&](Thread&, int t) { load_segrow(sh, t, nm >> 4, c); });
             const bool now = !pref;
-            run.phase([&](Thread& th, int t) { stage1_pre(th, sh, t, nm, now, c); });
+            run.phase([&](Thread& th, int t) {
+                stage1_pre(th, sh, t, nm, now, c);
+                keep_row_out(th, t, nm, c);
+            });
             queries(run, sh, nm, 0, c);
         }
+        if (c.d->keep) run.phase([&](Thread&, int t) { keep_factorials(t, c); });
     }
 
     // Canonical N_bad mod p of each query (fast/cuckoo_fast.cu:884-893).

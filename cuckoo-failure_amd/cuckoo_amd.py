@@ -41,6 +41,11 @@ class Instance(ctypes.Structure):
                 ("prime_off", ctypes.c_int64), ("prime_cnt", ctypes.c_int64)]
 
 
+class ServeStats(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("queries_per_s", ctypes.c_double), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("mean_batch", ctypes.c_double), ("batches", ctypes.c_long)]
+
+
 def lib():
     """The loaded engine library; raises if it has not been built."""
     global _lib
@@ -62,6 +67,26 @@ def lib():
                                    ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(u64)]
         L.cf_run_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Instance), ctypes.c_size_t, ctypes.POINTER(u64),
                                    ctypes.POINTER(u64)]
+        vp = ctypes.c_void_p
+        L.cf_tables_build.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(u64),
+                                      ctypes.c_size_t, ctypes.POINTER(vp)]
+        L.cf_tables_free.argtypes = [vp]
+        L.cf_tables_free.restype = None
+        L.cf_tables_bytes.argtypes = [vp]
+        L.cf_tables_bytes.restype = ctypes.c_size_t
+        L.cf_tables_need.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.cf_tables_need.restype = ctypes.c_long
+        L.cf_query.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t,
+                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(u64)]
+        L.cf_crt_log2.argtypes = [ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.c_size_t]
+        L.cf_crt_log2.restype = ctypes.c_double
+        L.cf_server_open.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+        L.cf_server_close.argtypes = [vp]
+        L.cf_server_close.restype = None
+        L.cf_server_slots.argtypes = [vp]
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.cf_server_replay.argtypes = [vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                       dp, dp, dp, ctypes.POINTER(u64), ctypes.POINTER(ServeStats)]
         L.cf_max_batch.restype = ctypes.c_long
         L.cf_max_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
         L.cf_mem_per_prime.restype = ctypes.c_size_t
@@ -101,6 +126,12 @@ def ntt_primes32(count, min_pow2):
     if lib().cf_ntt_primes32(count, min_pow2, out) != count:
         raise RuntimeError("not enough NTT-friendly primes")
     return list(out)[:count]
+
+
+def crt_log2(primes, res):
+    """log2 of the CRT combination of res modulo primes (host Garner); -inf for 0."""
+    k = len(primes)
+    return lib().cf_crt_log2((u64 * max(k, 1))(*primes), (u64 * max(k, 1))(*res), k)
 
 
 def mem_per_prime(nmax, Mmax, J=1):
@@ -177,3 +208,85 @@ class Engine:
 
     def max_batch(self, nmax, Mmax, J=1):
         return lib().cf_max_batch(self._h, nmax, Mmax, J)
+
+
+class Tables:
+    """A resident table set: suffix tables of every prime for rows 0..nmax, kept in device memory."""
+
+    def __init__(self, engine, d, nmax, Mmax, primes, model=0):
+        self.engine, self.primes = engine, list(primes)
+        self._h = ctypes.c_void_p()
+        P = (u64 * max(len(self.primes), 1))(*self.primes)
+        rc = lib().cf_tables_build(engine._h, d, model, nmax, Mmax, P, len(self.primes), ctypes.byref(self._h))
+        if rc != CF_OK:
+            raise CfError(rc, lib().cf_last_error(engine._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().cf_tables_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def bytes(self):
+        return lib().cf_tables_bytes(self._h)
+
+    def need(self, n, M):
+        return lib().cf_tables_need(self._h, n, M)
+
+    def query(self, queries, need=None):
+        """res[b][j] = N_bad(n_j, M_j) mod p_b for b < need[j] (None: all primes), else 0."""
+        J, np_ = len(queries), len(self.primes)
+        n = (ctypes.c_int32 * max(J, 1))(*[q[0] for q in queries])
+        M = (ctypes.c_int32 * max(J, 1))(*[q[1] for q in queries])
+        nd = None if need is None else (ctypes.c_int64 * max(J, 1))(*need)
+        res = (u64 * max(np_ * J, 1))()
+        rc = lib().cf_query(self._h, n, M, J, nd, res)
+        if rc != CF_OK:
+            raise CfError(rc, lib().cf_last_error(self.engine._h).decode())
+        return [[res[b * J + j] for j in range(J)] for b in range(np_)]
+
+
+class Server:
+    """The serving loop over a resident table set (continuous batching, host CRT)."""
+
+    def __init__(self, tables, slots=2, max_batch=64, crt_threads=8):
+        self.tables = tables
+        self._h = ctypes.c_void_p()
+        rc = lib().cf_server_open(tables._h, slots, max_batch, crt_threads, ctypes.byref(self._h))
+        if rc != CF_OK:
+            raise CfError(rc, lib().cf_last_error(tables.engine._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().cf_server_close(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def slots(self):
+        return lib().cf_server_slots(self._h)
+
+    def replay(self, n, M, arrival, residues=False):
+        """Replays requests in real time. Returns (latency_ms, log2_pfail, residues or None, stats)."""
+        k, np_ = len(n), len(self.tables.primes)
+        N = (ctypes.c_int32 * max(k, 1))(*[int(x) for x in n])
+        MM = (ctypes.c_int32 * max(k, 1))(*[int(x) for x in M])
+        A = (ctypes.c_double * max(k, 1))(*[float(x) for x in arrival])
+        lat = (ctypes.c_double * max(k, 1))()
+        l2 = (ctypes.c_double * max(k, 1))()
+        res = (u64 * max(k * np_, 1))() if residues else None
+        st = ServeStats()
+        rc = lib().cf_server_replay(self._h, k, N, MM, A, lat, l2, res, ctypes.byref(st))
+        if rc != CF_OK:
+            raise CfError(rc, lib().cf_last_error(self.tables.engine._h).decode())
+        rr = [[res[i * np_ + b] for b in range(np_)] for i in range(k)] if residues else None
+        return list(lat)[:k], list(l2)[:k], rr, st

@@ -89,6 +89,54 @@ typedef struct {
  * indices; indices no instance covers are left untouched. */
 int cf_run_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t* primes, uint64_t* res);
 
+/* ---- resident tables and serving ---- */
+typedef struct cf_tables cf_tables;
+
+/* Builds, and keeps in device memory, the one-table suffix tables of every prime (rows
+ * v = 0 .. nmax, factorials up to max(nmax, Mmax)): the reference's build_one_table_suffix_ntt
+ * (include/cuckoo/single_table.hpp:103-119) / the v-loop of fast/cuckoo_fast.cu:787-857, kept for
+ * later queries instead of being consumed. The set is admitted only if it fits in the free device
+ * memory (CF_E_NOMEM otherwise). Requires Mmax > nmax >= 1 and Mmax < every prime < 2^50. */
+int cf_tables_build(cf_engine* e, int d, int model, int nmax, int Mmax, const uint64_t* primes, size_t np,
+                    cf_tables** out);
+void cf_tables_free(cf_tables* t);
+/* Device bytes of the resident store. */
+size_t cf_tables_bytes(const cf_tables* t);
+/* The reference's need(n, M) for the set's d and model (fast/cuckoo_fast.cu:452-459). */
+long cf_tables_need(const cf_tables* t, int n, int M);
+
+/* Batched queries on a resident set: OneTableSuffix::nbad (single_table.hpp:178-229) /
+ * QueryAll with the host Horner finish (fast/cuckoo_fast.cu:306-323, 877-890) for J queries at
+ * once, each table row read once for the batch. res[b J + j] = N_bad(n_j, M_j) mod p_b for
+ * b < need[j] (need NULL: every prime), else 0. n_j <= nmax, n_j < M_j <= Mmax. */
+int cf_query(cf_tables* t, const int32_t* n, const int32_t* M, size_t J, const int64_t* need, uint64_t* res);
+
+/* log2 of the x in [0, prod p) with x = res[i] mod primes[i] (host CRT by Garner's mixed radix,
+ * as fast/post_gmp.py's CRT merge, in long double from the leading digits); -inf for x = 0. */
+double cf_crt_log2(const uint64_t* primes, const uint64_t* res, size_t k);
+
+/* The serving loop over one resident set. Requests are admitted into up to `slots` batches in
+ * flight at once (each on its own stream; fewer if device memory runs short), a free slot taking
+ * every request waiting at that moment (at most max_batch); finished requests are combined by CRT
+ * on crt_threads host threads. No counterpart in the reference, which has no serving mode. */
+typedef struct cf_server cf_server;
+typedef struct {
+    double wall_s;         /* first arrival to last completion */
+    double queries_per_s;
+    double p50_ms, p99_ms; /* arrival to completion (CRT included) */
+    double mean_batch;
+    long batches;
+} cf_serve_stats;
+int cf_server_open(cf_tables* t, int slots, int max_batch, int crt_threads, cf_server** out);
+void cf_server_close(cf_server* s);
+int cf_server_slots(const cf_server* s);
+/* Replays `count` requests (n[i], M[i]) arriving at arrival_s[i] seconds (non-decreasing) after
+ * the call starts, in real time, and returns when all are answered. Per request (any output may
+ * be NULL): latency_ms, log2_pfail = log2(N_bad / Q^n) over its need() primes, and res[i np + b]
+ * (N_bad mod p_b for b < need, else 0). CF_E_PRIMES if a request needs more primes than the set. */
+int cf_server_replay(cf_server* s, size_t count, const int32_t* n, const int32_t* M, const double* arrival_s,
+                     double* latency_ms, double* log2_pfail, uint64_t* res, cf_serve_stats* stats);
+
 /* How many primes of a table with nmax rows and factorials up to Mmax, answering J queries, fit
  * in the device memory now free (the reference's batch sizing, fast/cuckoo_fast.cu:742-750). */
 long cf_max_batch(cf_engine* e, int nmax, int Mmax, size_t J);

@@ -69,3 +69,23 @@ def test_engine_open_without_device_fails_loudly():
         pytest.skip("a HIP device may be present")
     with pytest.raises(cf.CfError):
         cf.Engine()
+
+
+def test_host_crt_log2_matches_exact():
+    """cf_crt_log2 (Garner over 50-bit primes, long double leading digits) against Python's exact CRT."""
+    import math
+    import random
+
+    rng = random.Random(3)
+    primes = cf.prime_range(0, 300)
+    for k in (1, 2, 5, 64, 300):
+        for _ in range(3):
+            res = [rng.randrange(p) for p in primes[:k]]
+            N, _ = oracle.crt(zip(primes[:k], res))
+            shift = max(N.bit_length() - 60, 0)
+            want = math.log2(N >> shift) + shift if N else -math.inf
+            got = cf.crt_log2(primes[:k], res)
+            assert abs(got - want) < 1e-12 * max(1.0, want), (k, got, want)
+    # small values, including zero and values below the first prime
+    assert cf.crt_log2(primes[:3], [0, 0, 0]) == -math.inf
+    assert cf.crt_log2(primes[:3], [5, 5, 5]) == math.log2(5)
