@@ -185,6 +185,7 @@ struct TaskDesc {
     int dd;              // d, the number of candidate slots per item
     int model;           // 0: uniform d-subset, g(v) = C(v, d); 1: with replacement, g(v) = v^d
     int pad_;
+    double wroot, wrooti;  // a task spread over workgroups: root of order 2^16 and its inverse
     double* keep;        // or null: resident table store of this prime (factorials, then the rows
                          // a_v[w] = V[v][w] / (w! v! (v-1-w)!) of v = v0+1 .. nm, packed by row)
 };
@@ -228,18 +229,21 @@ CF_HD Arena arena_of(double* base, const TaskDesc& t) {
     return a;
 }
 
-// A double at a byte offset below 4 GiB from a base pointer that is the same for the whole
-// workgroup: the per-entry address stays one 32-bit register.
-CF_HD double ldo(const double* base, uint32_t off) { return *(const double*)((const char*)base + off); }
+// A double at a byte offset from a base pointer that is the same for the whole workgroup: with a
+// 32-bit offset (below 4 GiB) the per-entry address stays one register.
+template <class O>
+CF_HD double ldo(const double* base, O off) { return *(const double*)((const char*)base + off); }
 CF_HD uint32_t ix8(int i) { return (uint32_t)i << 3; }
 
 // Byte offset of entry (r, c) (c < r) of a table from its pool base: tile (r/16, c/16) (its tile
 // number given) when the column block has been flushed, else the staging column c%16, which
-// starts st_off bytes after the pool base.
-CF_HD uint32_t tab_off(uint32_t st_off, int slot, int NS, int r, int c) {
+// starts st_off bytes after the pool base. O is uint32_t while the pools and staging of a table
+// span less than 4 GiB (tables of up to 16384 rows), uint64_t beyond.
+template <class O>
+CF_HD O tab_off(O st_off, int slot, int NS, int r, int c) {
     const int Y = r >> 4, X = c >> 4;
-    return X < Y ? ((uint32_t)slot << 11) + ((uint32_t)(r & 15) << 7) + ((uint32_t)(c & 15) << 3)
-                 : st_off + (((uint32_t)(c & 15) * (uint32_t)NS + (uint32_t)r) << 3);
+    return X < Y ? ((O)slot << 11) + ((O)(r & 15) << 7) + ((O)(c & 15) << 3)
+                 : st_off + (((O)(c & 15) * (O)NS + (O)r) << 3);
 }
 
 // Address of entry (r, c) of a table with rows r and columns c < r: tile (r/16, c/16) of the pool
@@ -277,17 +281,19 @@ struct Plan {
     static constexpr int NSEG = ((L + 1 + 15) / 16 + 15) / 16;  // segments per row block, nm <= L
 };
 
-// Shared (LDS) state of a workgroup.
-template <int LOG2L>
+// Shared (LDS) state of a workgroup running transforms of length 2^LOG2L on tables of up to
+// 2^CAP rows (CAP > LOG2L for a task spread over several workgroups).
+template <int LOG2L, int CAP = LOG2L>
 struct Shared {
     using P = Plan<LOG2L>;
+    using PC = Plan<CAP>;
     double buf[P::LDSW];
     double tw[2 * P::TWN + 2];
-    double lo_pre[64], lo_post[64], hi[P::HI];
+    double lo_pre[64], lo_post[64], hi[PC::HI];
     double red[P::T / 64 * 8 + 8];  // query partial sums: [wave][8]
     double red2[P::T / 64 + 1];     // per-wave sums of the stage-2 input
     double scal[8];                 // broadcast scalars
-    int segrow[P::NSEG + 1];        // segment slots of the current row block
+    int segrow[PC::NSEG + 1];       // segment slots of the current row block
 };
 
 // Per-thread state.
@@ -303,7 +309,8 @@ struct Engine {
     static constexpr int L = P::L, T = P::T;
 
     // ---- forward round 0 on registers (entries n1 T + t), result stored to LDS ----
-    static CF_HD void fwd0(Thread& th, Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m, const double* tw0) {
+    template <class SH>
+    static CF_HD void fwd0(Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m, const double* tw0) {
         reduce16(th.x, m);
         dft_groups<16>(th.x, d.W, m);
         twiddle_chain(th.x, tw0 + t, T, m);
@@ -311,7 +318,8 @@ struct Engine {
         for (int k = 0; k < 16; ++k) sh.buf[lpos(k * T + t)] = th.x[k];
     }
     // ---- forward round r (1 <= r < NTW) ----
-    static CF_HD void fwdr(int r, Thread& th, Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m) {
+    template <class SH>
+    static CF_HD void fwdr(int r, Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m) {
         const int lg = P::lgS(r), S = 1 << lg;
         const int b = t >> lg, n2 = t & (S - 1), base = (b << (lg + 4)) + n2;
         double(&x)[16] = th.x;
@@ -328,7 +336,8 @@ struct Engine {
     //      spectrum and run the inverse radix-RF ----
     // make_hat: store the spectrum (times 1/L) to hat; otherwise multiply by the spectrum entries
     // already fetched into th.pa.
-    static CF_HD void final_round(bool make_hat, Thread& th, Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m,
+    template <class SH>
+    static CF_HD void final_round(bool make_hat, Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m,
                                   double* hat) {
         double(&x)[16] = th.x;
 #pragma unroll
@@ -347,7 +356,8 @@ struct Engine {
         for (int i = 0; i < 16; ++i) sh.buf[lpos(16 * t + i)] = x[i];
     }
     // ---- inverse round r (1 <= r < NTW): twiddle w^-(n2 k), then inverse DFT16 ----
-    static CF_HD void invr(int r, Thread& th, Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m) {
+    template <class SH>
+    static CF_HD void invr(int r, Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m) {
         const int lg = P::lgS(r), S = 1 << lg;
         const int b = t >> lg, n2 = t & (S - 1), base = (b << (lg + 4)) + n2;
         double(&x)[16] = th.x;
@@ -362,7 +372,8 @@ struct Engine {
         for (int k = 0; k < 16; ++k) sh.buf[lpos(base + k * S)] = x[k];
     }
     // ---- inverse round 0: LDS -> registers (entries n1 T + t, natural order) ----
-    static CF_HD void inv0(Thread& th, Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m, const double* tw0) {
+    template <class SH>
+    static CF_HD void inv0(Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m, const double* tw0) {
         double(&x)[16] = th.x;
 #pragma unroll
         for (int k = 0; k < 16; ++k) x[k] = sh.buf[lpos(k * T + t)];
@@ -373,7 +384,8 @@ struct Engine {
 
     // ---- prologue pieces ----
     // round-0 twiddle bases and the LDS tables of rounds >= 1 (entries idx = t, t + T, ...)
-    static CF_HD void twiddles(Shared<LOG2L>& sh, int t, const TaskDesc& d, const Mod& m, double* tw0) {
+    template <class SH>
+    static CF_HD void twiddles(SH& sh, int t, const TaskDesc& d, const Mod& m, double* tw0) {
         double f = powmod(d.root, (uint64_t)t, m), i = powmod(d.rooti, (uint64_t)t, m);
         for (int k = 0; k < 4; ++k) {
             tw0[k * T + t] = f, tw0[(4 + k) * T + t] = i;

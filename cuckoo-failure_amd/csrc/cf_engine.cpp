@@ -87,9 +87,11 @@ int plan_task(cf_engine* e, uint64_t p, int d, int model, size_t nq, Q query, Ta
     if (tp.lg < 0)
         return fail(e, CF_E_UNSUPPORTED,
                     "p - 1 of " + std::to_string(p) + " has too few factors of two for n = " + std::to_string(tp.nm));
-    if (tp.lg > kMaxLog2L)
+    if (tp.lg > kMaxLog2L && tp.lg != kWideLog2)
         return fail(e, CF_E_UNSUPPORTED, "n = " + std::to_string(tp.nm) + " needs a transform longer than 2^" +
-                                             std::to_string(kMaxLog2L) + ", which this build does not run");
+                                             std::to_string(kWideLog2) + ", which this build does not run");
+    if (tp.lg == kWideLog2 && tp.queries.size() > 8)
+        return fail(e, CF_E_UNSUPPORTED, "more than 8 queries on one table of more than 16384 rows");
     tp.bytes = task_bytes(plan_for(e, tp.nm), tp.F1, (int)tp.queries.size(), tp.lg);
     // two convolutions of length L per row, plus the tile traffic
     tp.cost = 2.0 * tp.nm * ((double)(1 << tp.lg) * (tp.lg + 1)) + 8.0 * tp.nm * (double)tp.nm;
@@ -108,7 +110,8 @@ void describe(cf_engine* e, const std::vector<TaskPlan>& tasks, size_t at, size_
         const TaskPlan& tp = tasks[k];
         TaskDesc& t = td[k - at];
         memset(&t, 0, sizeof t);
-        task_constants(tp.p, tp.lg, t);
+        if (tp.lg == kWideLog2) wide_constants(tp.p, t);
+        else task_constants(tp.p, tp.lg, t);
         const SlotPlan& sp = plan_for(e, tp.nm);
         auto so = slot_off.find(tp.nm);
         if (so == slot_off.end()) {
@@ -144,16 +147,18 @@ struct Uploaded {
     const int* slots;
     uint64_t* res;
     unsigned* next;
+    unsigned* sync;
 };
 
 int upload(cf_engine* e, const std::vector<TaskDesc>& td, const std::vector<QueryDesc>& qd, const std::vector<int>& slots,
-           size_t nres, Uploaded& u, const std::vector<QueueJob>* jobs = nullptr) {
+           size_t nres, Uploaded& u, const std::vector<QueueJob>* jobs = nullptr, size_t nsync = 0) {
     auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t njobs = jobs ? jobs->size() : 0;
     const size_t o_jb = 0, o_td = o_jb + up(sizeof(QueueJob) * njobs), o_qd = o_td + up(sizeof(TaskDesc) * td.size());
     const size_t o_sl = o_qd + up(sizeof(QueryDesc) * qd.size());
     const size_t o_nx = o_sl + up(sizeof(int) * slots.size());
-    const size_t o_rs = o_nx + 256;
+    const size_t o_sy = o_nx + 256;
+    const size_t o_rs = o_sy + up(sizeof(unsigned) * nsync);
     const size_t small = o_rs + sizeof(uint64_t) * std::max<size_t>(nres, 1);
     int rc = ensure(e, &e->small, &e->small_cap, small, "the task descriptors");
     if (rc) return rc;
@@ -165,6 +170,8 @@ int upload(cf_engine* e, const std::vector<TaskDesc>& td, const std::vector<Quer
     HIP_TRY(e, hipMemcpyAsync(sm + o_qd, qd.data(), sizeof(QueryDesc) * qd.size(), hipMemcpyHostToDevice, e->stream), "upload");
     HIP_TRY(e, hipMemcpyAsync(sm + o_sl, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, e->stream), "upload");
     HIP_TRY(e, hipMemcpyAsync(sm + o_nx, &zero, sizeof zero, hipMemcpyHostToDevice, e->stream), "upload");
+    if (nsync) HIP_TRY(e, hipMemsetAsync(sm + o_sy, 0, sizeof(unsigned) * nsync, e->stream), "hipMemsetAsync");
+    u.sync = (unsigned*)(sm + o_sy);
     u.jobs = (const QueueJob*)(sm + o_jb);
     u.td = (const TaskDesc*)(sm + o_td);
     u.qd = (const QueryDesc*)(sm + o_qd);
@@ -265,7 +272,11 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
                                            std::to_string(budget) + " usable");
         size_t cls_end = end;
         while (cls_end < tasks.size() && tasks[cls_end].lg == lg) ++cls_end;
-        const size_t fit = end - at, left = cls_end - at, cap = (size_t)std::max(resident_tasks(lg, e->device), 1);
+        const bool wide = lg == kWideLog2;
+        const size_t cap = wide ? (size_t)std::min(kWideMaxTasks, std::max(wide_resident(e->device), 0) / 16)
+                                : (size_t)std::max(resident_tasks(lg, e->device), 1);
+        if (!cap) return fail(e, CF_E_UNSUPPORTED, "the device cannot hold the 16 workgroups of a 2^16 transform");
+        const size_t fit = std::min(end - at, wide ? cap : end - at), left = cls_end - at;
         size_t take = fit >= cap ? fit / cap * cap : fit;
         if (take < left) {
             const size_t launches = (left + take - 1) / take;
@@ -280,13 +291,19 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
         int rc = ensure(e, &e->arena, &e->arena_cap, bytes, "the task tables");
         if (rc) return rc;
         Uploaded u;
-        rc = upload(e, td, qd, slots, nres, u);
+        rc = upload(e, td, qd, slots, nres, u, nullptr, wide ? 2 * (size_t)nt + 2 : 0);
         if (rc) return rc;
         float ms = 0;
         rc = timed_launch(e, [&] {
+            if (wide) return launch_wide(nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, u.sync, e->stream);
             return launch_tasks(lg, nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, e->stream);
         }, qd, u, nres, res, ms);
         if (rc) return rc;
+        if (wide) {
+            unsigned aborted = 0;
+            HIP_TRY(e, hipMemcpy(&aborted, u.sync + 2 * nt, sizeof aborted, hipMemcpyDeviceToHost), "download");
+            if (aborted) return fail(e, CF_E_DEVICE, "a group barrier of the 2^16 transform timed out");
+        }
         kernel_ms_total += ms;
         e->stats.launches++;
         e->stats.tasks += nt;
@@ -471,7 +488,7 @@ int cf_ntt_primes32(int count, int min_pow2, uint32_t* out) {
 
 int cf_engine_log2_length(int nmax, uint64_t p) {
     const int lg = engine_log2_length(nmax, p);
-    return lg > kMaxLog2L ? -1 : lg;
+    return lg > kMaxLog2L && lg != kWideLog2 ? -1 : lg;
 }
 
 }  // extern "C"

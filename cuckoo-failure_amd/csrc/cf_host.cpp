@@ -102,6 +102,7 @@ int two_adic_valuation(uint64_t x) {
 int engine_log2_length(int nm, uint64_t p) {
     int lg = 10;
     while ((1 << lg) < nm) ++lg;
+    if (lg == 15) lg = 16;  // tables of 16385 .. 32768 rows also run on the 2^16 transform
     if (lg > two_adic_valuation(p - 1)) return -1;
     return lg;
 }
@@ -124,6 +125,19 @@ void task_constants(uint64_t p, int log2L, TaskDesc& d) {
         d.W[i] = balanced(powmod_u64(w16, i, p), p);
         d.Wi[i] = balanced(powmod_u64(w16i, i, p), p);
     }
+}
+
+// A task spread over workgroups: inner constants of length 4096 (from the same non-residue, so
+// the inner root is the 16th power of the order-2^16 root), 1/L of the whole 2^16 transform.
+void wide_constants(uint64_t p, TaskDesc& d) {
+    task_constants(p, 12, d);
+    const uint64_t L = uint64_t(1) << 16;
+    uint64_t x = 2;
+    while (powmod_u64(x, (p - 1) / 2, p) != p - 1) ++x;
+    const uint64_t w = powmod_u64(x, (p - 1) / L, p), wi = powmod_u64(w, L - 1, p);
+    d.wroot = balanced(w, p);
+    d.wrooti = balanced(wi, p);
+    d.linv = balanced(powmod_u64(L % p, p - 2, p), p);
 }
 
 SlotPlan plan_slots(int nm) {
@@ -153,7 +167,9 @@ SlotPlan plan_slots(int nm) {
 }
 
 size_t task_bytes(const SlotPlan& sp, int F1, int q_cnt, int log2L) {
-    return 8 * (size_t)arena_doubles(sp.pool, sp.NS, F1, q_cnt, 1 << log2L);
+    // a 2^16 task also holds its exchange rows, outer twiddles and per-workgroup sums
+    const int64_t extra = log2L == 16 ? 65536 + 32768 + 16 * 8 + 16 + 16 : 0;
+    return 8 * (size_t)(arena_doubles(sp.pool, sp.NS, F1, q_cnt, 1 << log2L) + extra);
 }
 
 }  // namespace cf

@@ -5,6 +5,7 @@
 
 #include "cf_kernels.hpp"
 #include "cf_task.hpp"
+#include "cf_wide.hpp"
 
 namespace cf {
 
@@ -63,6 +64,7 @@ struct DevRunner {
 };
 
 // The task's context from its descriptor; arena_base is where its arena offset counts from.
+template <int LG>
 __device__ __forceinline__ Ctx make_ctx(const TaskDesc* d, const QueryDesc* qs, const int* slots, double* arena_base) {
     Ctx c;
     c.d = d;
@@ -74,6 +76,9 @@ __device__ __forceinline__ Ctx make_ctx(const TaskDesc* d, const QueryDesc* qs, 
     c.model = d->model;
     c.dd = d->dd;
     c.v0 = d->model ? 1 : d->dd;
+    c.es = Plan<LG>::T;
+    c.eb = 0;
+    c.L = Plan<LG>::L;
     return c;
 }
 
@@ -88,7 +93,7 @@ __global__ void __launch_bounds__(Plan<LG>::T / kVirtual<LG>)
     __shared__ Shared<LG> sh;
     DevRunner<VT, TP> run;
     run.t = (int)threadIdx.x;
-    const Ctx c = make_ctx(tasks + blockIdx.x, qs, slots, mem);
+    const Ctx c = make_ctx<LG>(tasks + blockIdx.x, qs, slots, mem);
     Task<LG>::run_task(run, sh, c);
     for (int j = threadIdx.x; j < c.d->q_cnt; j += TP) res[qs[c.d->q_off + j].out] = (uint64_t)Task<LG>::result(j, c);
 }
@@ -118,7 +123,7 @@ __device__ __forceinline__ void queue_job(QueueLds& lds, const QueueJob& jb, con
     const int k = __builtin_amdgcn_readfirstlane(jb.first + (g < jb.cnt ? g : 0));
     DevRunner<1, T> run;
     run.t = (int)threadIdx.x - g * T;
-    const Ctx c = make_ctx(tasks + k, qs, slots, arena + slot_stride * g);
+    const Ctx c = make_ctx<LG>(tasks + k, qs, slots, arena + slot_stride * g);
     Task<LG>::run_task(run, sh, c);
     if (g < jb.cnt)
         for (int j = run.t; j < c.d->q_cnt; j += T) res[qs[c.d->q_off + j].out] = (uint64_t)Task<LG>::result(j, c);
@@ -160,6 +165,85 @@ hipError_t launch_queue(int workgroups, int njobs, const QueueJob* jobs, const T
     if (njobs <= 0 || workgroups <= 0) return hipSuccess;
     k_queue<<<dim3((unsigned)workgroups), kQueueThreads, 0, stream>>>(jobs, njobs, tasks, qs, slots, mem, arena_stride, res, next);
     return hipGetLastError();
+}
+
+// ---- tasks spread over kWideG workgroups (transforms of 2^16) ----
+// Group barrier of the workgroups of one task: each arrives on a monotone counter and waits until
+// all have arrived. The wait is bounded: a waiter that gives up sets the launch's abort flag,
+// which releases every other waiter, and the task ends with an error instead of hanging.
+constexpr long kWideSpinLimit = 1L << 24;
+
+struct WideRunner : DevRunner<1, kWideT> {
+    unsigned* cnt;    // this task's arrival counter
+    unsigned* abort;  // the launch's abort flag
+    int* flag;        // LDS: the outcome for the whole workgroup
+    unsigned target = 0;
+    __device__ __forceinline__ bool gsync() {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            target += kWideG;
+            __threadfence();  // this workgroup's writes (made complete by the barrier) to the device
+            __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int bad = 0;
+            long spins = 0;
+            while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    bad = 1;
+                    break;
+                }
+                if (++spins > kWideSpinLimit) {
+                    __hip_atomic_fetch_or(abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    bad = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            __threadfence();  // the other workgroups' writes, not stale cache lines
+            *flag = bad;
+        }
+        __syncthreads();
+        return *flag != 0;
+    }
+};
+
+// Workgroup -> (task, member): workgroups are dealt to the 8 XCDs round robin, so the kWideG
+// members of a task get numbers 8 apart and share one XCD (and its L2) for their exchanges.
+// sync: [2 ntasks] arrival counters (zero at launch), then the abort flag.
+__global__ void __launch_bounds__(kWideT)
+    k_wide(const TaskDesc* __restrict__ tasks, int ntasks, const QueryDesc* __restrict__ qs, const int* __restrict__ slots,
+           double* __restrict__ mem, uint64_t* __restrict__ res, unsigned* __restrict__ sync) {
+    __shared__ Wide::Sh sh;
+    __shared__ int flag;
+    const int L = (int)blockIdx.x, xcd = L & 7, i = L >> 3;
+    const int task = xcd + 8 * (i / kWideG), g = i % kWideG;
+    if (task >= ntasks) return;
+    const TaskDesc* d = tasks + task;
+    WideRunner run;
+    run.t = (int)threadIdx.x;
+    run.cnt = sync + 2 * task;
+    run.abort = sync + 2 * ntasks;
+    run.flag = &flag;
+    const Ctx c = Wide::ctx(d, qs, slots, mem, g, d->model, d->dd);
+    const WideArena w = wide_arena_of(mem, *d);
+    const bool ok = Wide::run_task(run, sh, c, w, g);
+    if (ok && g == 0)
+        for (int j = threadIdx.x; j < d->q_cnt; j += kWideT) res[qs[d->q_off + j].out] = (uint64_t)Wide::TK::result(j, c);
+}
+
+int wide_resident(int device) {
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, device) != hipSuccess) return 0;
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_wide, kWideT, 0) != hipSuccess) return 0;
+    return pr.multiProcessorCount * per;
+}
+
+hipError_t launch_wide(int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem, uint64_t* res,
+                       unsigned* sync, hipStream_t stream) {
+    if (ntasks <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)(((ntasks + 7) / 8) * 8 * kWideG);
+    void* args[] = {(void*)&tasks, (void*)&ntasks, (void*)&qs, (void*)&slots, (void*)&mem, (void*)&res, (void*)&sync};
+    return hipLaunchCooperativeKernel((const void*)k_wide, dim3(grid), dim3(kWideT), args, 0, stream);
 }
 
 // Workgroups of the log2L kernel that the device runs at the same time (CUs x blocks per CU).

@@ -29,4 +29,12 @@ int queue_resident(int device);
 hipError_t launch_queue(int workgroups, int njobs, const QueueJob* jobs, const TaskDesc* tasks, const QueryDesc* qs,
                         const int* slots, double* mem, int64_t arena_stride, uint64_t* res, unsigned* next, hipStream_t stream);
 
+// Tasks with transforms of 2^16, each spread over 16 workgroups that meet at group barriers: one
+// cooperative launch of ntasks tasks (all workgroups resident at once). sync holds 2 ntasks + 2
+// zeros at launch; afterwards sync[2 ntasks] != 0 means a barrier timed out (results invalid).
+constexpr int kWideLog2 = 16, kWideMaxTasks = 16;
+int wide_resident(int device);  // workgroups of the wide kernel the device holds at once
+hipError_t launch_wide(int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem, uint64_t* res,
+                       unsigned* sync, hipStream_t stream);
+
 }  // namespace cf

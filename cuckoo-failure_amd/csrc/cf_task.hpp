@@ -12,6 +12,8 @@
 // form described in cf_core.hpp); the query reduction is :306-323 with the host Horner finish of
 // :877-890 done on the device, row by row as the rows become final.
 #pragma once
+#include <type_traits>
+
 #include "cf_core.hpp"
 
 namespace cf {
@@ -26,13 +28,17 @@ struct Ctx {
     int model;  // 0: g(v) = C(v, dd); 1: g(v) = v^dd
     int dd;     // d, the number of candidate slots per item
     int v0;     // first v with g(v) != 0
+    // Element map: register k of thread t holds transform entry k es + eb + t (a one-workgroup
+    // task: es = T, eb = 0; workgroup g of a wide task: es = 4096, eb = 256 g); L the task's
+    // transform length.
+    int es, eb, L;
 };
 
-template <int LOG2L>
+template <int LOG2L, int CAP = LOG2L>
 struct Task {
     using P = Plan<LOG2L>;
     using E = Engine<LOG2L>;
-    using Sh = Shared<LOG2L>;
+    using Sh = Shared<LOG2L, CAP>;
     static constexpr int L = P::L, T = P::T, NW = T / 64;  // NW waves
 
     // ---------------- prologue ----------------
@@ -78,11 +84,11 @@ struct Task {
         for (int i = lo; i < hi; ++i) c.a.ifact[i] = reduce(mulred(c.a.ifact[i], f, c.m), c.m);
     }
     static CF_HD void inv_table(int t, const Ctx& c) {
-        for (int i = t; i < c.d->F1; i += T) c.a.inv[i] = i ? reduce(mulred(c.a.ifact[i], c.a.fact[i - 1], c.m), c.m) : 0.0;
+        for (int i = c.eb + t; i < c.d->F1; i += c.es) c.a.inv[i] = i ? reduce(mulred(c.a.ifact[i], c.a.fact[i - 1], c.m), c.m) : 0.0;
     }
     static CF_HD void zero_staging(int t, const Ctx& c) {
         const int64_t n = 16 * (int64_t)c.d->NS;
-        for (int64_t i = t; i < n; i += T) c.a.Vst[i] = 0.0, c.a.Sst[i] = 0.0;
+        for (int64_t i = c.eb + t; i < n; i += c.es) c.a.Vst[i] = 0.0, c.a.Sst[i] = 0.0;
     }
     static CF_HD double g_of(int v, const Ctx& c) {
         if (c.model) return powmod((double)v, (uint64_t)c.dd, c.m);
@@ -99,7 +105,7 @@ struct Task {
     static CF_HD void kernel_input(Thread& th, int t, const Ctx& c) {
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int e = k * T + t;
+            const int e = k * c.es + c.eb + t;
             th.x[k] = (e >= 1 && e < c.d->nm) ? c.a.inv[e] : 0.0;
         }
     }
@@ -118,16 +124,18 @@ struct Task {
     // Entries (r, e) of a table for this thread's e in [lo, hi), as raw loads into dst (other
     // entries get the value at a clamped, valid address and must be masked by the consumer).
     // Row block r/16 must be the one whose segment row is in LDS.
+    // byte offsets within a table: 32-bit while it spans less than 4 GiB (up to 2^14 rows)
+    using Off = typename std::conditional<(CAP > 14), uint64_t, uint32_t>::type;
     static CF_HD void fetch_row(double (&dst)[16], const double* pool, const double* st, const Sh& sh, int t, int r, int lo,
                                 int hi, const Ctx& c) {
         const int Y = r >> 4;
-        const uint32_t st_off = (uint32_t)((st - pool) << 3);
+        const Off st_off = (Off)((st - pool) << 3);
         const int NS = c.d->NS;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int e = k * T + t, ce = (e >= lo && e < hi) ? e : 0, X = ce >> 4;
+            const int e = k * c.es + c.eb + t, ce = (e >= lo && e < hi) ? e : 0, X = ce >> 4;
             const int tile = X < Y ? tile_of(sh, X) : 0;
-            dst[k] = ldo(pool, tab_off(st_off, tile, NS, r, ce));
+            dst[k] = ldo(pool, tab_off<Off>(st_off, tile, NS, r, ce));
         }
     }
     static CF_HD void fetch_hat(Thread& th, int t, const Ctx& c) {
@@ -140,7 +148,7 @@ struct Task {
         const int nm = c.d->nm;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int e = k * T + t, p = (off && e == 0) ? L : e;
+            const int e = k * c.es + c.eb + t, p = (off && e == 0) ? c.L : e;
             const bool ok = off ? (p > v && p <= nm) : (e >= v && e < nm);
             th.pa[k] = ldo(c.a.ifact, ix8(ok ? p - v - off : 0));
         }
@@ -152,7 +160,7 @@ struct Task {
         const double iv = c.a.inv[v], hand = sh.scal[1];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int e = k * T + t;
+            const int e = k * c.es + c.eb + t;
             const double x = e < v - 1 ? th.pa[k] : hand;
             const double a = mulred(x, iv, c.m);
             th.x[k] = e < v ? a : 0.0;
@@ -168,7 +176,7 @@ struct Task {
                 double iv[16];
 #pragma unroll
                 for (int k = 0; k < 16; ++k) {
-                    const int e = k * T + t;
+                    const int e = k * c.es + c.eb + t;
                     iv[k] = ldo(c.a.inv, ix8(M - (e < v ? e : 0)));
                 }
 #pragma unroll
@@ -205,19 +213,19 @@ struct Task {
         double y[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int e = k * T + t;
+            const int e = k * c.es + c.eb + t;
             y[k] = mulred(th.x[k], mulred(th.pa[k], th.pa[k], c.m), c.m);
             if (e >= v && e < nm) col[e] = y[k];
         }
         fetch_row(th.pa, c.a.Spool, c.a.Sst, sh, t, v, 1, v, c);
 #pragma unroll
         for (int k = 0; k < 16; ++k)
-            if (k * T + t == v) th.pa[k] = y[k];
+            if (k * c.es + c.eb + t == v) th.pa[k] = y[k];
     }
     // Column v of a staging area set to zero on rows lo..hi-1.
     static CF_HD void zero_col(double* st, int t, int v, int lo, int hi, const Ctx& c) {
         double* col = st + (int64_t)(v & 15) * c.d->NS;
-        for (int r = lo + t; r < hi; r += T) col[r] = 0.0;
+        for (int r = lo + c.eb + t; r < hi; r += c.es) col[r] = 0.0;
     }
     // Power tables of g for step v: lo_pre[i] = v! g^i, lo_post[i] = g^i / v!, hi[j] = g^(64 j).
     static CF_HD void powtab(Sh& sh, int t, int v, double g, const Ctx& c) {
@@ -241,7 +249,7 @@ struct Task {
         double bsum = 0.0;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int e = k * T + t, ok = e >= 1 && e <= v, ce = ok ? e : v;
+            const int e = k * c.es + c.eb + t, ok = e >= 1 && e <= v, ce = ok ? e : v;
             const double b = ok ? mulred(th.pa[k], gpow(sh.lo_pre, sh, v - ce, c.m), c.m) : 0.0;
             bsum += b;
             if ((k & 3) == 3) bsum = reduce(bsum, c.m);  // four terms below 4.1 p
@@ -258,7 +266,7 @@ struct Task {
         const double s1 = gpow(sh.lo_post, sh, v, c.m) - B;  // g^v / v! - B
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int e = k * T + t, p = e ? e : L;  // c'[p mod L]: entry 0 stands for p = L
+            const int e = k * c.es + c.eb + t, p = e ? e : c.L;  // c'[p mod L]: entry 0 stands for p = L
             const bool ok = p > v && p <= nm;
             const int j = ok ? p - v : 1;
             const double fj = mulred(gpow(sh.lo_post, sh, j, c.m), mulred(th.pa[k], th.pa[k], c.m), c.m);
@@ -280,17 +288,18 @@ struct Task {
         const int* tl = (const int*)sh.buf;
         const int n = (c.d->NB - 1 - X) * 256;
         const uint32_t NS = (uint32_t)c.d->NS, row0 = 16 * (uint32_t)(X + 1);
-        for (int base = 0; base < n; base += 16 * T) {
+        const int gt = c.eb + t, GT = c.es;  // this thread among all threads of the task
+        for (int base = 0; base < n; base += 16 * GT) {
             double y[16];
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
-                const int idx = base + u * T + t, ci = idx < n ? idx : 0;
+                const int idx = base + u * GT + gt, ci = idx < n ? idx : 0;
                 const uint32_t j = (uint32_t)ci >> 8, col = ((uint32_t)ci >> 4) & 15, r = (uint32_t)ci & 15;
                 y[u] = ldo(st, (col * NS + row0 + 16 * j + r) << 3);
             }
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
-                const int idx = base + u * T + t;
+                const int idx = base + u * GT + gt;
                 if (idx < n) {
                     const int col = (idx >> 4) & 15, r = idx & 15;
                     pool[(int64_t)tl[idx >> 8] * 256 + (r << 4) + col] = y[u];
@@ -346,14 +355,14 @@ struct Task {
         double* row = keep + keep_row(c.d->F1, v);
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int e = k * T + t;
+            const int e = k * c.es + c.eb + t;
             if (e < v) row[e] = th.x[k];
         }
     }
     static CF_HD void keep_factorials(int t, const Ctx& c) {
         double* keep = c.d->keep;
         const int F1 = c.d->F1;
-        for (int i = t; i < F1; i += T) {
+        for (int i = c.eb + t; i < F1; i += c.es) {
             keep[i] = c.a.fact[i];
             keep[F1 + i] = c.a.ifact[i];
             keep[2 * F1 + i] = c.a.inv[i];

@@ -1,0 +1,326 @@
+// cf_wide.hpp -- one prime on kWideG workgroups: the suffix-table recurrence with transforms of
+// length L = 2^16, which do not fit one CU's LDS.
+//
+// The transform is split four-step: entry e = 4096 n1 + n2 (n1 < 16, n2 < 4096). Workgroup g owns
+// the columns n2 = 256 g + t (t < 256): its thread t holds the 16 entries 4096 k + 256 g + t in
+// registers, which is also the element map of every stage function (Ctx::es = 4096,
+// Ctx::eb = 256 g). A convolution is then
+//   outer: DFT16 over n1 in registers, twiddle w^(n2 k1) (w of order 2^16), store row k1 of the
+//          exchange buffer (16 x 4096 doubles in HBM);
+//   group barrier;
+//   inner: workgroup g loads row k1 = g and runs the one-workgroup engine of length 4096
+//          (Engine<12>: rounds in LDS, pointwise product with its slice of the kernel spectrum,
+//          inverse rounds), storing the row back;
+//   group barrier;
+//   outer inverse: load the thread's column, twiddle w^-(n2 k1), inverse DFT16.
+// Sums over all entries (query terms, the stage-2 block sum) are reduced per workgroup, written
+// to small per-workgroup slots and added up after the next group barrier. The factorial tables
+// are built by workgroup 0; staging zeroing and the tile flush are split over all workgroups.
+//
+// Reference correspondence is that of cf_task.hpp (fast/cuckoo_fast.cu:787-898); only the
+// transform's distribution differs.
+#pragma once
+#include "cf_task.hpp"
+
+namespace cf {
+
+constexpr int kWideG = 16;            // workgroups per prime
+constexpr int kWideLog2L = 16;        // L = 2^16
+constexpr int kWideInner = 12;        // inner transform length 2^12 = 4096
+constexpr int kWideT = 256;           // threads per workgroup
+constexpr int kWideQ = 8;             // queries per wide task (at most)
+
+// Extra arena of a wide task, after the one-workgroup layout of length 2^16 (doubles).
+struct WideArena {
+    double* exch;   // 16 x 4096 exchange rows
+    double* twout;  // per workgroup 2048: w^(n2 2^k) (k < 4) then w^-(n2 2^k), at k 256 + t
+    double* qpart;  // [g][kWideQ] per-workgroup query sums
+    double* bpart;  // [g] per-workgroup stage-2 sums
+    double* hand;   // [0]: the hand-over entry when it crosses workgroups
+};
+CF_HD int64_t wide_extra_doubles() { return 65536 + 32768 + kWideG * kWideQ + kWideG + 16; }
+CF_HD WideArena wide_arena_of(double* base, const TaskDesc& t) {
+    WideArena w;
+    w.exch = base + t.arena + arena_doubles(t.pool, t.NS, t.F1, t.q_cnt, 1 << kWideLog2L);
+    w.twout = w.exch + 65536;
+    w.qpart = w.twout + 32768;
+    w.bpart = w.qpart + kWideG * kWideQ;
+    w.hand = w.bpart + kWideG;
+    return w;
+}
+
+struct Wide {
+    using TK = Task<kWideInner, kWideLog2L>;  // stage functions (wide element map), LDS layout
+    using Sh = Shared<kWideInner, kWideLog2L>;
+    using E = Engine<kWideInner>; // the inner transform
+    using P = Plan<kWideInner>;
+    static constexpr int T = kWideT, G = kWideG, NW = T / 64, N2 = 4096;
+    static_assert(P::T == T, "inner plan");
+
+    // The context of workgroup g: the wide element map, its slice of the spectrum and of the
+    // inner round-0 twiddles.
+    static CF_HD Ctx ctx(const TaskDesc* d, const QueryDesc* qs, const int* slots, double* mem, int g, int model, int dd) {
+        Ctx c;
+        c.d = d;
+        c.m.p = d->p;
+        c.m.pinv = d->pinv;
+        c.a = arena_of(mem, *d);
+        c.a.hat += (int64_t)N2 * g;
+        c.a.tw0 += 2048 * (int64_t)g;
+        c.slots = slots + d->slot_off;
+        c.q = qs;
+        c.model = model;
+        c.dd = dd;
+        c.v0 = model ? 1 : dd;
+        c.es = N2;
+        c.eb = 256 * g;
+        c.L = 1 << kWideLog2L;
+        return c;
+    }
+
+    // ---- the outer step ----
+    static CF_HD void outer_twiddles(const WideArena& w, int g, int t, const TaskDesc& d, const Mod& m) {
+        double* tw = w.twout + 2048 * g;
+        const uint64_t n2 = (uint64_t)(256 * g + t);
+        double f = powmod(d.wroot, n2, m), i = powmod(d.wrooti, n2, m);
+        for (int k = 0; k < 4; ++k) {
+            tw[k * 256 + t] = f, tw[1024 + k * 256 + t] = i;
+            f = reduce(mulred(f, f, m), m), i = reduce(mulred(i, i, m), m);
+        }
+    }
+    static CF_HD void outer_fwd(Thread& th, const WideArena& w, int g, int t, const TaskDesc& d, const Mod& m) {
+        reduce16(th.x, m);
+        dft_groups<16>(th.x, d.W, m);
+        twiddle_chain(th.x, w.twout + 2048 * g + t, 256, m);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) w.exch[k * N2 + 256 * g + t] = th.x[k];
+    }
+    static CF_HD void outer_inv(Thread& th, const WideArena& w, int g, int t, const TaskDesc& d, const Mod& m) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) th.x[k] = w.exch[k * N2 + 256 * g + t];
+        twiddle_chain(th.x, w.twout + 2048 * g + 1024 + t, 256, m);
+        reduce16(th.x, m);
+        dft_groups<16>(th.x, d.Wi, m);
+    }
+    // inner row g: exchange -> registers (entries 256 k + t) and back
+    static CF_HD void inner_load(Thread& th, const WideArena& w, int g, int t) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) th.x[k] = w.exch[g * N2 + 256 * k + t];
+    }
+    static CF_HD void inner_store(const Thread& th, const WideArena& w, int g, int t) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) w.exch[g * N2 + 256 * k + t] = th.x[k];
+    }
+
+    // The inner part of a convolution (between the two group barriers). The factorials of the
+    // stage output (off: 0 stage 1, 1 stage 2) are fetched into th.pa on the way.
+    template <class R>
+    static CF_HD void inner(R& run, Sh& sh, const WideArena& w, int g, int v, int off, bool make_hat,
+                            const Ctx& c) {
+        run.phase([&](Thread& th, int t) {
+            inner_load(th, w, g, t);
+            E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
+        });
+        for (int r = 1; r < P::NTW; ++r)
+            run.phase([&](Thread& th, int t) {
+                if (!make_hat && r == P::NTW - 1) TK::fetch_hat(th, t, c);
+                E::fwdr(r, th, sh, t, *c.d, c.m);
+            });
+        run.phase([&](Thread& th, int t) { E::final_round(make_hat, th, sh, t, *c.d, c.m, c.a.hat); });
+        if (make_hat) return;
+        for (int r = P::NTW - 1; r >= 1; --r)
+            run.phase([&](Thread& th, int t) {
+                if (r == 1) TK::fetch_ifact(th, t, v, off, c);
+                E::invr(r, th, sh, t, *c.d, c.m);
+            });
+        run.phase([&](Thread& th, int t) {
+            E::inv0(th, sh, t, *c.d, c.m, c.a.tw0);
+            inner_store(th, w, g, t);
+        });
+    }
+
+    // Query terms of row v (th.x, stage-1 input form) for queries 0 .. cnt-1: this workgroup's
+    // wave sums to LDS (inside the caller's phase), then to the workgroup's slots.
+    template <class R>
+    static CF_HD void query_wave(R& run, Thread& th, Sh& sh, int t, int v, int cnt, const Ctx& c) {
+        if (cnt)
+            run.wave_partial8(sh.red, th, t, [&](Thread& th2, int t2, double (&acc)[8]) {
+                TK::query_partial(th2, t2, v, 0, cnt, acc, c);
+                return cnt;
+            }, c.m);
+    }
+    static CF_HD void query_slots(const Sh& sh, const WideArena& w, int g, int t, int cnt, const Mod& m) {
+        if (t < cnt) {
+            double s = 0.0;
+            for (int wv = 0; wv < NW; ++wv) s += sh.red[wv * 8 + t];  // 4 values below p/2 + 4
+            w.qpart[g * kWideQ + t] = reduce(s, m);
+        }
+    }
+    // Workgroup 0 finishes the queries from all workgroups' slots (after a group barrier).
+    static CF_HD void query_total(const WideArena& w, int g, int t, int v, int cnt, const Ctx& c) {
+        if (g == 0 && t < cnt) {
+            double s = 0.0;
+            for (int h = 0; h < G; ++h) {
+                s += w.qpart[h * kWideQ + t];
+                if ((h & 3) == 3) s = reduce(s, c.m);
+            }
+            TK::query_update(t, v, 0, reduce(s, c.m), c);
+        }
+    }
+
+    template <class R>
+    static CF_HD bool run_task(R& run, Sh& sh, const Ctx& c, const WideArena& w, int g) {
+        const int nm = c.d->nm;
+        const TaskDesc& d = *c.d;
+        // ---- prologue: workgroup 0 builds the factorial tables ----
+        run.phase([&](Thread&, int t) {
+            TK::zero_staging(t, c);
+            if (g == 0) TK::fact_local(sh, t, c);
+        });
+        if (g == 0) {
+            int src = 0;
+            for (int s = 1; s < T; s <<= 1, src ^= 1) run.phase([&](Thread&, int t) { TK::scan_step(sh, t, s, src, c); });
+            run.phase([&](Thread&, int t) { TK::fact_fix(sh, t, src, c); });
+            run.phase([&](Thread&, int t) {
+                if (t == 0) sh.scal[0] = powmod(c.a.fact[d.F1 - 1], (uint64_t)d.p - 2, c.m);
+            });
+            run.phase([&](Thread&, int t) { TK::ifact_local(sh, t, c); });
+            src = 0;
+            for (int s = 1; s < T; s <<= 1, src ^= 1) run.phase([&](Thread&, int t) { TK::scan_step(sh, t, s, src, c); });
+            run.phase([&](Thread&, int t) { TK::ifact_fix(sh, t, src, c); });
+        }
+        if (run.gsync()) return false;
+        run.phase([&](Thread&, int t) {
+            TK::inv_table(t, c);
+            E::twiddles(sh, t, d, c.m, c.a.tw0);
+            outer_twiddles(w, g, t, d, c.m);
+            if (g == 0) TK::query_init(t, c);
+        });
+        if (run.gsync()) return false;
+        // kernel spectrum (times 1/L), this workgroup's slice
+        run.phase([&](Thread& th, int t) {
+            TK::kernel_input(th, t, c);
+            outer_fwd(th, w, g, t, d, c.m);
+        });
+        if (run.gsync()) return false;
+        inner(run, sh, w, g, 0, 0, true, c);
+
+        // ---- the v-loop ----
+        int segY = -1;
+        bool pref = false;
+        for (int v = 1; v < nm; ++v) {
+            const bool s1 = v > c.v0, s2 = v >= c.v0;
+            const double gv = s2 ? TK::g_of(v, c) : 0.0;
+            if ((v >> 4) != segY && (s1 || s2)) {
+                segY = v >> 4;
+                run.phase([&](Thread&, int t) { TK::load_segrow(sh, t, segY, c); });
+            }
+            if (s1) {
+                const int nact = TK::active_queries(v, c);  // <= kWideQ
+                const bool now = !pref;
+                run.phase([&](Thread& th, int t) {
+                    TK::stage1_pre(th, sh, t, v, now, c);
+                    query_wave(run, th, sh, t, v, nact, c);
+                    outer_fwd(th, w, g, t, d, c.m);
+                });
+                if (nact) run.phase([&](Thread&, int t) { query_slots(sh, w, g, t, nact, c.m); });
+                if (run.gsync()) return false;
+                inner(run, sh, w, g, v, 0, false, c);
+                if (run.gsync()) return false;
+                run.phase([&](Thread& th, int t) {
+                    query_total(w, g, t, v, nact, c);
+                    outer_inv(th, w, g, t, d, c.m);
+                    TK::stage1_post(th, sh, t, v, c);
+                    if (s2) TK::powtab(sh, t, v, gv, c);
+                });
+            } else {
+                run.phase([&](Thread&, int t) {
+                    TK::zero_col(c.a.Sst, t, v, v, nm, c);
+                    if (s2) TK::powtab(sh, t, v, gv, c);
+                });
+            }
+            const bool next = s2 && (v & 15) != 15;
+            if (s2) {
+                if (v > c.v0) {
+                    run.phase([&](Thread& th, int t) {
+                        const double bpart = TK::stage2_pre(th, sh, t, v, c);
+                        run.wave_partial1(sh.red2, th, t, bpart, c.m);
+                        outer_fwd(th, w, g, t, d, c.m);
+                    });
+                    run.phase([&](Thread&, int t) {
+                        if (t == 0) {
+                            double s = 0.0;
+                            for (int wv = 0; wv < NW; ++wv) s += sh.red2[wv];
+                            w.bpart[g] = reduce(s, c.m);
+                        }
+                    });
+                    if (run.gsync()) return false;
+                    inner(run, sh, w, g, v, 1, false, c);
+                    if (run.gsync()) return false;
+                    run.phase([&](Thread& th, int t) {
+                        double B = 0.0;
+                        for (int h = 0; h < G; ++h) {
+                            B += w.bpart[h];
+                            if ((h & 3) == 3) B = reduce(B, c.m);
+                        }
+                        outer_inv(th, w, g, t, d, c.m);
+                        TK::stage2_post(th, sh, t, v, true, reduce(B, c.m), c);
+                        if (next) TK::fetch_next(th, sh, t, v, c);
+                    });
+                } else {
+                    run.phase([&](Thread& th, int t) {
+                        TK::fetch_ifact(th, t, v, 1, c);
+                        TK::stage2_post(th, sh, t, v, false, 0.0, c);
+                        if (next) TK::fetch_next(th, sh, t, v, c);
+                    });
+                }
+            } else {
+                run.phase([&](Thread&, int t) {
+                    TK::zero_col(c.a.Vst, t, v, v + 1, nm + 1, c);
+                    if (t == 0) sh.scal[1] = 0.0;
+                });
+            }
+            pref = next;
+            // a step without convolutions (v <= v0) had no group barrier, but its staging writes
+            // (zeroed columns, the stage-2 output at v = v0) are spread over all workgroups
+            if (!s1 && !(s2 && v > c.v0) && run.gsync()) return false;
+            // the hand-over entry (v+1, v) was made by the owner of entry v+1; when the owner of
+            // entry v (which uses it next step) is another workgroup, it goes through memory
+            const bool cross = ((v + 1) & 255) == 0;
+            if (cross) {
+                run.phase([&](Thread&, int t) {
+                    if (((v + 1) & (N2 - 1)) / 256 == g && t == 0) w.hand[0] = sh.scal[1];
+                });
+                if (run.gsync()) return false;
+                run.phase([&](Thread&, int t) {
+                    if (t == 0) sh.scal[1] = w.hand[0];
+                });
+            }
+            if ((v & 15) == 15) {
+                if (!cross && run.gsync()) return false;  // every workgroup's staging columns
+                run.phase([&](Thread&, int t) { TK::flush_tiles(sh, t, v >> 4, c); });
+                run.phase([&](Thread&, int t) {
+                    TK::flush(sh, c.a.Vpool, c.a.Vst, t, v >> 4, c);
+                    TK::flush(sh, c.a.Spool, c.a.Sst, t, v >> 4, c);
+                });
+                if (run.gsync()) return false;  // the tiles, before any row fetch
+            }
+        }
+        // row nm of V only feeds the queries with n = nm
+        if (nm > c.v0) {
+            if ((nm >> 4) != segY) run.phase([&](Thread&, int t) { TK::load_segrow(sh, t, nm >> 4, c); });
+            const bool now = !pref;
+            const int nact = TK::active_queries(nm, c);
+            run.phase([&](Thread& th, int t) {
+                TK::stage1_pre(th, sh, t, nm, now, c);
+                query_wave(run, th, sh, t, nm, nact, c);
+            });
+            if (nact) run.phase([&](Thread&, int t) { query_slots(sh, w, g, t, nact, c.m); });
+            if (run.gsync()) return false;
+            run.phase([&](Thread&, int t) { query_total(w, g, t, nm, nact, c); });
+        }
+        return !run.gsync();  // workgroup 0's query totals are final
+    }
+};
+
+}  // namespace cf

@@ -8,6 +8,7 @@ the programs' outputs are kept, as data.
     python tests/golden/make_golden.py            # everything except the n = 65536 prime
     python tests/golden/make_golden.py --big      # also n = 65536 (about 32 GB RAM, ~3 min)
     python tests/golden/make_golden.py --only uneven   # just the uneven-batch fixture
+    python tests/golden/make_golden.py --only wide     # just the n > 16384 fixture (~2 min, 8 cores)
 """
 import json
 import os
@@ -121,6 +122,16 @@ def fast_big(n, M, first, count):
             "stdout": "".join(r["stdout"] for r in recs)}
 
 
+def fast_wide():
+    """Tables longer than 16384 rows (transforms of 2^15 and 2^16): a few primes each."""
+    cases = [(3, [(16385, 16386)], 0, 2), (3, [(20000, 40000), (17000, 20000)], 0, 3), (2, [(24000, 30000)], 5, 2),
+             (3, [(32768, 65536)], 0, 2), (4, [(40000, 41000)], 1, 1)]
+    jobs = [(d, qs, i) for d, qs, first, count in cases for i in range(first, first + count)]
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count())) as ex:
+        recs = list(ex.map(lambda j: fast_run(j[0], j[1], j[2], 1), jobs))
+    return [{"d": d, "queries": [list(q) for q in qs], "prime": i, "stdout": r["stdout"]} for (d, qs, i), r in zip(jobs, recs)]
+
+
 def fail_cases():
     """The reference's 32-bit CLI (src/cuckoo_fail.cpp), ntt algorithm."""
     args = [
@@ -193,9 +204,14 @@ def uneven_golden(n_max=512):
 
 def main():
     os.makedirs(HERE, exist_ok=True)
-    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "uneven":
+    only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
+    if only == "uneven":
         with open(os.path.join(HERE, "uneven_batch.json"), "w") as f:
             json.dump(uneven_golden(), f, indent=0)
+        return
+    if only == "wide":
+        with open(os.path.join(HERE, "fast_wide.json"), "w") as f:
+            json.dump(fast_wide(), f, indent=0)
         return
     rc, so, _ = run([CHECK])
     assert rc == 0, "reference check program failed"
@@ -213,6 +229,8 @@ def main():
         json.dump(fast_big(16384, 32768, 0, 8), f, indent=0)
     with open(os.path.join(HERE, "uneven_batch.json"), "w") as f:
         json.dump(uneven_golden(), f, indent=0)
+    with open(os.path.join(HERE, "fast_wide.json"), "w") as f:
+        json.dump(fast_wide(), f, indent=0)
     if "--big" in sys.argv:
         with open(os.path.join(HERE, "fast_65536.json"), "w") as f:
             json.dump(fast_big(65536, 131072, 0, 1), f, indent=0)

@@ -97,5 +97,5 @@ def test_batch_edges(eng):
         eng.run_batch([(3, 0, 20, 60, [59])])  # prime not above M
     assert ei.value.code == cf.CF_E_ARGS
     with pytest.raises(cf.CfError) as ei:
-        eng.run_batch([(3, 0, 20000, 40000, P[:1])])  # transform beyond 2^14
+        eng.run_batch([(3, 0, 70000, 140000, P[:1])])  # transform beyond 2^16
     assert ei.value.code == cf.CF_E_UNSUPPORTED

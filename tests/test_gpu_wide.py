@@ -1,0 +1,51 @@
+"""Tables of more than 16384 rows: transforms of 2^16 spread over 16 workgroups per prime.
+
+Pinned against the reference's own residues (its host build, one prime per run) for n from 16385
+to 65536, several queries per table, d = 2, 3, 4, and a batch of tables in one launch."""
+import pytest
+
+import cuckoo_amd as cf
+from golden_data import load, residue_lines
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = cf.Engine()
+    yield e
+    e.close()
+
+
+def test_wide_tables_vs_reference(eng):
+    recs = load("fast_wide.json")
+    # group the reference's single-prime runs by (d, queries) into one engine call each
+    groups = {}
+    for r in recs:
+        key = (r["d"], tuple(tuple(q) for q in r["queries"]))
+        groups.setdefault(key, []).extend(residue_lines(r["stdout"]))
+    for (d, qs), lines in groups.items():
+        primes = sorted({(i, p) for i, p, _, _, _ in lines})
+        print(f"d={d} queries={qs} primes={[i for i, _ in primes]}", flush=True)
+        got = eng.run(list(qs), [p for _, p in primes], d)
+        st = eng.stats()
+        assert st.log2_length == 16
+        for i, p, n, M, r in lines:
+            b = [x for x, _ in primes].index(i)
+            assert got[b][list(qs).index((n, M))] == r, (d, i, n, M)
+
+
+def test_wide_n65536_reference_prime0(eng):
+    rec = load("fast_65536.json")
+    (i, p, n, M, r), = residue_lines(rec["stdout"])
+    got = eng.run([(n, M)], [p], 3)
+    assert got[0][0] == r
+
+
+def test_wide_mixed_with_short_tables(eng):
+    # one call with tables of both kinds: each prime's table is as long as its longest query
+    recs = load("fast_wide.json")
+    rec = next(r for r in recs if r["d"] == 3 and [list(q) for q in r["queries"]] == [[16385, 16386]])
+    (i, p, n, M, r), = residue_lines(rec["stdout"])
+    got = eng.run([(n, M), (200, 400)], [p, 1125899865948161], 3, mask=[[1, 0], [0, 1]])
+    assert got[0][0] == r and got[1][1] == 918297158731096
