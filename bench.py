@@ -164,6 +164,40 @@ def bench_serve(a):
     }))
 
 
+def bench_wide(a):
+    """One table of n = 65536, M = 131072, d = 3 per prime over a slice of the 50-bit list: transforms
+    of 2^16 spread over 16 workgroups per prime, 16 primes per launch (device memory)."""
+    import cuckoo_amd as cf
+
+    n, M, d = 65536, 131072, 3
+    count = a.primes if a.primes != NPRIMES else 256
+    primes = cf.prime_range(0, count)
+    eng = cf.Engine()
+    t0 = time.perf_counter()
+    got = eng.run([(n, M)], primes, d)
+    wall = time.perf_counter() - t0
+    st = eng.stats()
+    dev = eng.device_name()
+    eng.close()
+    assert got[0][0] == 740179115061037, "prime 0 differs from the reference's residue"
+    per_launch_ms = st.kernel_ms / st.launches
+    mm_per_launch = work_modmul(n, d) * st.tasks / st.launches
+    achieved = mm_per_launch * FLOP_PER_MODMUL / (per_launch_ms * 1e-3) / 1e12
+    print(json.dumps({
+        "metric": "prime-residues/s at n=65536,d=3 (bit-exact)",
+        "value": round(count / wall, 4), "unit": "prime-residues/s", "n_gpus": 1, "higher_is_better": True,
+        "ms_per_step": round(wall * 1e3, 1), "dtype": "f64 (exact integer residues < 2^50)",
+        "data": "synthetic (deterministic: the reference's 50-bit prime list)",
+        "config": {"workload": f"one-table suffix-table build + nbad, n={n}, M={M}, d={d}, {count}-prime slice 0..{count - 1}",
+                   "device": dev, "launches": st.launches, "primes_per_launch": st.tasks // max(st.launches, 1),
+                   "device_bytes": st.device_bytes},
+        "roofline": {"bound": "valu", "kernel": "k_wide", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "launch_ms": round(per_launch_ms, 1), "modmul_per_launch": mm_per_launch},
+        "reference_check": "prime 0 residue equals the reference host build's (740179115061037)",
+    }))
+
+
 def tabs_bytes_gb(np_, nmax, Mmax):
     F1 = max(nmax, Mmax) + 1
     return np_ * 8 * (3 * F1 + nmax * (nmax + 1) // 2) / 1e9
@@ -176,9 +210,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--primes", type=int, default=NPRIMES)
     ap.add_argument("--cpu-baseline", default="ifma", choices=["ifma", "scalar", "none"])
-    ap.add_argument("--workload", default="fast", choices=["fast", "uneven", "serve"],
+    ap.add_argument("--workload", default="fast", choices=["fast", "uneven", "serve", "wide"],
                     help="fast: the headline n=16384 batch; uneven: the 256-instance work-queue batch; "
-                         "serve: resident tables and a replayed request stream")
+                         "serve: resident tables and a replayed request stream; wide: n=65536 over 256 primes")
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--queries", type=int, default=100000)
     ap.add_argument("--rate", type=float, default=1500.0)
@@ -196,6 +230,8 @@ def main():
         return bench_uneven(a)
     if a.workload == "serve":
         return bench_serve(a)
+    if a.workload == "wide":
+        return bench_wide(a)
     import cuckoo_amd as cf
     eng = cf.Engine()
     primes = cf.prime_range(0, a.primes)
