@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -197,6 +198,13 @@ int timed_launch(cf_engine* e, L launch, const std::vector<QueryDesc>& qd, const
     return CF_OK;
 }
 
+// XCD-local group barriers for the 2^16 kernel, unless CF_WIDE_DEVICE_BARRIERS=1 asks for
+// device-scope ones everywhere.
+int wide_local() {
+    const char* v = getenv("CF_WIDE_DEVICE_BARRIERS");
+    return !(v && v[0] == '1');
+}
+
 // Runs every task. With queue, the tasks whose transform fits the work-queue kernel go to one
 // persistent launch (longest first); the rest, and everything without queue, run as launches
 // of one transform length each, packed into device memory.
@@ -291,11 +299,11 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
         int rc = ensure(e, &e->arena, &e->arena_cap, bytes, "the task tables");
         if (rc) return rc;
         Uploaded u;
-        rc = upload(e, td, qd, slots, nres, u, nullptr, wide ? 2 * (size_t)nt + 2 : 0);
+        rc = upload(e, td, qd, slots, nres, u, nullptr, wide ? wide_sync_words(nt) : 0);
         if (rc) return rc;
         float ms = 0;
         rc = timed_launch(e, [&] {
-            if (wide) return launch_wide(nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, u.sync, e->stream);
+            if (wide) return launch_wide(nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, u.sync, wide_local(), e->stream);
             return launch_tasks(lg, nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, e->stream);
         }, qd, u, nres, res, ms);
         if (rc) return rc;

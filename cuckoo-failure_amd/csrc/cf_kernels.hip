@@ -178,11 +178,13 @@ struct WideRunner : DevRunner<1, kWideT> {
     unsigned* abort;  // the launch's abort flag
     int* flag;        // LDS: the outcome for the whole workgroup
     unsigned target = 0;
+    bool local = false;  // all workgroups of the task share one XCD (and L2)
     __device__ __forceinline__ bool gsync() {
+        if (local) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's stores are in L2
         __syncthreads();
         if (threadIdx.x == 0) {
             target += kWideG;
-            __threadfence();  // this workgroup's writes (made complete by the barrier) to the device
+            if (!local) __threadfence();  // this workgroup's writes (made complete by the barrier) to the device
             __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             int bad = 0;
             long spins = 0;
@@ -198,7 +200,10 @@ struct WideRunner : DevRunner<1, kWideT> {
                 }
                 __builtin_amdgcn_s_sleep(2);
             }
-            __threadfence();  // the other workgroups' writes, not stale cache lines
+            // the other workgroups' writes, not stale cache lines: within one XCD only this CU's
+            // L1 can be stale (the L2 is shared); otherwise the device-scope fence
+            if (local) asm volatile("buffer_inv sc0" ::: "memory");
+            else __threadfence();
             *flag = bad;
         }
         __syncthreads();
@@ -206,12 +211,21 @@ struct WideRunner : DevRunner<1, kWideT> {
     }
 };
 
+__device__ __forceinline__ unsigned xcc_id() {
+    unsigned x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+    return x;
+}
+
 // Workgroup -> (task, member): workgroups are dealt to the 8 XCDs round robin, so the kWideG
-// members of a task get numbers 8 apart and share one XCD (and its L2) for their exchanges.
-// sync: [2 ntasks] arrival counters (zero at launch), then the abort flag.
+// members of a task get numbers 8 apart and share one XCD (and its L2) for their exchanges. Each
+// workgroup records the XCD it runs on; after the first (device-scope) group barrier every member
+// checks the record, and only when all kWideG share one XCD (and `local` is allowed) do the
+// remaining barriers use the XCD-local form.
+// sync: [2 ntasks] arrival counters (zero at launch), the abort flag, then kWideG XCD ids per task.
 __global__ void __launch_bounds__(kWideT)
     k_wide(const TaskDesc* __restrict__ tasks, int ntasks, const QueryDesc* __restrict__ qs, const int* __restrict__ slots,
-           double* __restrict__ mem, uint64_t* __restrict__ res, unsigned* __restrict__ sync) {
+           double* __restrict__ mem, uint64_t* __restrict__ res, unsigned* __restrict__ sync, int local) {
     __shared__ Wide::Sh sh;
     __shared__ int flag;
     const int L = (int)blockIdx.x, xcd = L & 7, i = L >> 3;
@@ -223,6 +237,14 @@ __global__ void __launch_bounds__(kWideT)
     run.cnt = sync + 2 * task;
     run.abort = sync + 2 * ntasks;
     run.flag = &flag;
+    unsigned* xcc = sync + 2 * ntasks + 2 + kWideG * task;
+    if (threadIdx.x == 0) xcc[g] = xcc_id();
+    if (run.gsync()) return;
+    if (local) {
+        bool same = true;
+        for (int h = 1; h < kWideG; ++h) same &= xcc[h] == xcc[0];
+        run.local = same;  // the same data for every member: one decision per task
+    }
     const Ctx c = Wide::ctx(d, qs, slots, mem, g, d->model, d->dd);
     const WideArena w = wide_arena_of(mem, *d);
     const bool ok = Wide::run_task(run, sh, c, w, g);
@@ -239,10 +261,11 @@ int wide_resident(int device) {
 }
 
 hipError_t launch_wide(int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem, uint64_t* res,
-                       unsigned* sync, hipStream_t stream) {
+                       unsigned* sync, int local, hipStream_t stream) {
     if (ntasks <= 0) return hipSuccess;
     const unsigned grid = (unsigned)(((ntasks + 7) / 8) * 8 * kWideG);
-    void* args[] = {(void*)&tasks, (void*)&ntasks, (void*)&qs, (void*)&slots, (void*)&mem, (void*)&res, (void*)&sync};
+    void* args[] = {(void*)&tasks, (void*)&ntasks, (void*)&qs, (void*)&slots, (void*)&mem, (void*)&res, (void*)&sync,
+                    (void*)&local};
     return hipLaunchCooperativeKernel((const void*)k_wide, dim3(grid), dim3(kWideT), args, 0, stream);
 }
 
