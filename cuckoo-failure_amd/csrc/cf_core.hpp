@@ -203,7 +203,7 @@ struct QueryDesc {
 // Layout of a task's arena (doubles), in this order.
 struct Arena {
     double *Vpool, *Spool;  // pool tiles of 16 x 16 entries, row-major
-    double *Vst, *Sst;      // staging: 16 columns x NS rows, column-major
+    double *Vst, *Sst;      // staging: 2 blocks (column block parity) x 16 columns x NS rows, column-major
     double *fact, *ifact, *inv;
     double* qtot;  // per query: Horner accumulator, then Q = g(M)
     double* hat;   // kernel spectrum times 1/L, in the final round's order (L entries)
@@ -211,7 +211,7 @@ struct Arena {
 };
 
 CF_HD int64_t arena_doubles(int pool, int NS, int F1, int q_cnt, int L) {
-    return (int64_t)pool * 512 + 32 * (int64_t)NS + 3 * (int64_t)F1 + 2 * (int64_t)q_cnt + L + L / 2;
+    return (int64_t)pool * 512 + 64 * (int64_t)NS + 3 * (int64_t)F1 + 2 * (int64_t)q_cnt + L + L / 2;
 }
 
 CF_HD Arena arena_of(double* base, const TaskDesc& t) {
@@ -219,8 +219,8 @@ CF_HD Arena arena_of(double* base, const TaskDesc& t) {
     a.Vpool = base + t.arena;
     a.Spool = a.Vpool + (int64_t)t.pool * 256;
     a.Vst = a.Spool + (int64_t)t.pool * 256;
-    a.Sst = a.Vst + 16 * (int64_t)t.NS;
-    a.fact = a.Sst + 16 * (int64_t)t.NS;
+    a.Sst = a.Vst + 32 * (int64_t)t.NS;
+    a.fact = a.Sst + 32 * (int64_t)t.NS;
     a.ifact = a.fact + t.F1;
     a.inv = a.ifact + t.F1;
     a.qtot = a.inv + t.F1;
@@ -235,15 +235,18 @@ template <class O>
 CF_HD double ldo(const double* base, O off) { return *(const double*)((const char*)base + off); }
 CF_HD uint32_t ix8(int i) { return (uint32_t)i << 3; }
 
+// Staging column of column v: block parity (v/16 mod 2), then column v%16 (in doubles).
+CF_HD int64_t st_col(int v, int NS) { return (int64_t)((((v >> 4) & 1) << 4) + (v & 15)) * NS; }
+
 // Byte offset of entry (r, c) (c < r) of a table from its pool base: tile (r/16, c/16) (its tile
-// number given) when the column block has been flushed, else the staging column c%16, which
-// starts st_off bytes after the pool base. O is uint32_t while the pools and staging of a table
+// number given) when the column block has been flushed, else its staging column, the staging
+// starting st_off bytes after the pool base. O is uint32_t while the pools and staging of a table
 // span less than 4 GiB (tables of up to 16384 rows), uint64_t beyond.
 template <class O>
 CF_HD O tab_off(O st_off, int slot, int NS, int r, int c) {
     const int Y = r >> 4, X = c >> 4;
     return X < Y ? ((O)slot << 11) + ((O)(r & 15) << 7) + ((O)(c & 15) << 3)
-                 : st_off + (((O)(c & 15) * (O)NS + (O)r) << 3);
+                 : st_off + (((((O)(X & 1) << 4) + (O)(c & 15)) * (O)NS + (O)r) << 3);
 }
 
 // Address of entry (r, c) of a table with rows r and columns c < r: tile (r/16, c/16) of the pool
@@ -251,7 +254,7 @@ CF_HD O tab_off(O st_off, int slot, int NS, int r, int c) {
 // Branch-free, so a phase can issue all its loads before waiting for any.
 CF_HD const double* tab_addr(const double* pool, const double* st, int slot, int NS, int r, int c) {
     const int Y = r >> 4, X = c >> 4;
-    return X < Y ? pool + (int64_t)slot * 256 + ((r & 15) << 4) + (c & 15) : st + (int64_t)(c & 15) * NS + r;
+    return X < Y ? pool + (int64_t)slot * 256 + ((r & 15) << 4) + (c & 15) : st + st_col(c, NS) + r;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -279,6 +282,7 @@ struct Plan {
     static constexpr int TWN = tw_off(NTW);  // entries per direction
     static constexpr int HI = (L >> 6) + 2;  // power-table entries g^(64 j), j <= nm/64
     static constexpr int NSEG = ((L + 1 + 15) / 16 + 15) / 16;  // segments per row block, nm <= L
+    static constexpr int NBMAX = (L + 1 + 15) / 16;               // row blocks, nm <= L
 };
 
 // Shared (LDS) state of a workgroup running transforms of length 2^LOG2L on tables of up to
@@ -294,6 +298,7 @@ struct Shared {
     double red2[P::T / 64 + 1];     // per-wave sums of the stage-2 input
     double scal[8];                 // broadcast scalars
     int segrow[PC::NSEG + 1];       // segment slots of the current row block
+    unsigned short dseg[P::NBMAX];  // segments of the tiles whose flush is deferred (one block)
 };
 
 // Per-thread state.

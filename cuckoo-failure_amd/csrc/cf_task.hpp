@@ -87,7 +87,7 @@ struct Task {
         for (int i = c.eb + t; i < c.d->F1; i += c.es) c.a.inv[i] = i ? reduce(mulred(c.a.ifact[i], c.a.fact[i - 1], c.m), c.m) : 0.0;
     }
     static CF_HD void zero_staging(int t, const Ctx& c) {
-        const int64_t n = 16 * (int64_t)c.d->NS;
+        const int64_t n = 32 * (int64_t)c.d->NS;
         for (int64_t i = c.eb + t; i < n; i += c.es) c.a.Vst[i] = 0.0, c.a.Sst[i] = 0.0;
     }
     static CF_HD double g_of(int v, const Ctx& c) {
@@ -208,7 +208,7 @@ struct Task {
     // with the factorials in th.pa (fetched ahead). Then row v of S2 is fetched into th.pa for the
     // stage-2 input of the next phase; its entry (v, v) is the value this thread just computed.
     static CF_HD void stage1_post(Thread& th, const Sh& sh, int t, int v, const Ctx& c) {
-        double* col = c.a.Sst + (int64_t)(v & 15) * c.d->NS;
+        double* col = c.a.Sst + st_col(v, c.d->NS);
         const int nm = c.d->nm;
         double y[16];
 #pragma unroll
@@ -224,7 +224,7 @@ struct Task {
     }
     // Column v of a staging area set to zero on rows lo..hi-1.
     static CF_HD void zero_col(double* st, int t, int v, int lo, int hi, const Ctx& c) {
-        double* col = st + (int64_t)(v & 15) * c.d->NS;
+        double* col = st + st_col(v, c.d->NS);
         for (int r = lo + c.eb + t; r < hi; r += c.es) col[r] = 0.0;
     }
     // Power tables of g for step v: lo_pre[i] = v! g^i, lo_post[i] = g^i / v!, hi[j] = g^(64 j).
@@ -261,7 +261,7 @@ struct Task {
     // the staging column v % 16 (factorials in th.pa, B the block sum). The entry of row v+1 also
     // goes to the LDS hand-over for the next step's stage-1 input.
     static CF_HD void stage2_post(const Thread& th, Sh& sh, int t, int v, bool conv, double B, const Ctx& c) {
-        double* col = c.a.Vst + (int64_t)(v & 15) * c.d->NS;
+        double* col = c.a.Vst + st_col(v, c.d->NS);
         const int nm = c.d->nm;
         const double s1 = gpow(sh.lo_post, sh, v, c.m) - B;  // g^v / v! - B
 #pragma unroll
@@ -287,7 +287,7 @@ struct Task {
     static CF_HD void flush(const Sh& sh, double* pool, const double* st, int t, int X, const Ctx& c) {
         const int* tl = (const int*)sh.buf;
         const int n = (c.d->NB - 1 - X) * 256;
-        const uint32_t NS = (uint32_t)c.d->NS, row0 = 16 * (uint32_t)(X + 1);
+        const uint32_t NS = (uint32_t)c.d->NS, row0 = 16 * (uint32_t)(X + 1), cb = (uint32_t)(X & 1) << 4;
         const int gt = c.eb + t, GT = c.es;  // this thread among all threads of the task
         for (int base = 0; base < n; base += 16 * GT) {
             double y[16];
@@ -295,7 +295,7 @@ struct Task {
             for (int u = 0; u < 16; ++u) {
                 const int idx = base + u * GT + gt, ci = idx < n ? idx : 0;
                 const uint32_t j = (uint32_t)ci >> 8, col = ((uint32_t)ci >> 4) & 15, r = (uint32_t)ci & 15;
-                y[u] = ldo(st, (col * NS + row0 + 16 * j + r) << 3);
+                y[u] = ldo(st, ((cb + col) * NS + row0 + 16 * j + r) << 3);
             }
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
@@ -308,22 +308,105 @@ struct Task {
         }
     }
 
+    // ---- deferred flush (transforms with at least three radix-16 rounds, one workgroup) ----
+    // At the end of column block X only tile (X+1, X), which the next step reads, is written
+    // (flush_head). The tiles (Y, X), Y >= X+2, are first read in block X+2: they are moved during
+    // the 16 steps of block X+1, in 64 slices, by the first forward and the last inverse round of
+    // each convolution, whose th.pa is free (loads issued one phase earlier, after the stage
+    // input or the pointwise product has used th.pa; stores at the end of the round);
+    // the staging is double-buffered by block parity, so block X+1 writes the other half.
+    static constexpr bool kDefer = P::NTW >= 3 && CAP == LOG2L;
+    static CF_HD void flush_head(int t, int X, const Ctx& c) {
+        if (X + 1 >= c.d->NB) return;
+        const int NS = c.d->NS, cb = (X & 1) << 4;
+        const int64_t tile = (int64_t)c.slots[(X + 1) * c.d->NSEG + (X >> 4)] * 16 + (X & 15);
+        for (int gi = t; gi < 512; gi += T) {
+            const int tbl = gi >> 8, col = (gi >> 4) & 15, r = gi & 15;
+            const double* st = tbl ? c.a.Sst : c.a.Vst;
+            double* pool = tbl ? c.a.Spool : c.a.Vpool;
+            pool[tile * 256 + r * 16 + col] = st[(int64_t)(cb + col) * NS + 16 * (X + 1) + r];
+        }
+    }
+    // segment numbers of the deferred tiles (X+2+j, X) into LDS (at the start of block X+1)
+    static CF_HD void defer_list(Sh& sh, int t, int X, const Ctx& c) {
+        const int n = c.d->NB - 2 - X, NSEG = c.d->NSEG;
+        for (int j = t; j < n; j += T) sh.dseg[j] = (unsigned short)c.slots[(X + 2 + j) * NSEG + (X >> 4)];
+    }
+    // Slice k (of 64) of block X's deferred entries (both tables): entry gi of the slice range is
+    // table gi / half, tile j = idx >> 8, column (idx >> 4) & 15, row 16 (X+2+j) + (idx & 15).
+    struct DeferSlice {
+        int lo, hi, half;
+    };
+    static CF_HD DeferSlice defer_slice(int X, int k, const Ctx& c) {
+        const int half = (c.d->NB - 2 - X) * 256, E = half > 0 ? 2 * half : 0, chunk = (E + 63) / 64;
+        DeferSlice d;
+        d.lo = k * chunk < E ? k * chunk : E;
+        d.hi = d.lo + chunk < E ? d.lo + chunk : E;
+        d.half = half;
+        return d;
+    }
+    static CF_HD void defer_load(Thread& th, int t, int X, int k, const Ctx& c) {
+        const DeferSlice d = defer_slice(X, k, c);
+        if (d.lo >= d.hi) return;
+        const uint32_t NS = (uint32_t)c.d->NS, cb = (uint32_t)(X & 1) << 4;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int gi = d.lo + u * T + t, g = gi < d.hi ? gi : d.lo;
+            const int tbl = g >= d.half, idx = g - (tbl ? d.half : 0);
+            const uint32_t j = (uint32_t)idx >> 8, col = ((uint32_t)idx >> 4) & 15, r = (uint32_t)idx & 15;
+            th.pa[u] = ldo(tbl ? c.a.Sst : c.a.Vst, ((cb + col) * NS + 16 * ((uint32_t)X + 2 + j) + r) << 3);
+        }
+    }
+    static CF_HD void defer_store(const Thread& th, const Sh& sh, int t, int X, int k, const Ctx& c) {
+        const DeferSlice d = defer_slice(X, k, c);
+        if (d.lo >= d.hi) return;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int gi = d.lo + u * T + t;
+            if (gi < d.hi) {
+                const int tbl = gi >= d.half, idx = gi - (tbl ? d.half : 0);
+                const int j = idx >> 8, col = (idx >> 4) & 15, r = idx & 15;
+                const int64_t tile = (int64_t)sh.dseg[j] * 16 + (X & 15);
+                (tbl ? c.a.Spool : c.a.Vpool)[tile * 256 + r * 16 + col] = th.pa[u];
+            }
+        }
+    }
+    // slices k0 .. k1-1 in a phase of their own (a step of block X+1 without a convolution)
+    template <class R>
+    static CF_HD void defer_phase(R& run, Sh& sh, int X, int k0, int k1, const Ctx& c) {
+        run.phase([&](Thread& th, int t) {
+            for (int k = k0; k < k1; ++k) {
+                defer_load(th, t, X, k, c);
+                defer_store(th, sh, t, X, k, c);
+            }
+        });
+    }
+
     // ---------------- the convolution, split at its first and last phase ----------------
     // The caller fuses its own work into the first phase (before fwd0) and the last (after inv0).
     // The middle phases also fetch ahead: the kernel spectrum for the final round, and the
     // factorials 1/(e - v - off)! that the caller's last phase needs.
+    // With kDefer and X >= 0, the first forward and the last inverse round also carry the
+    // deferred-flush slices k and k + 1 of block X.
     template <class R>
-    static CF_HD void conv_middle(R& run, Sh& sh, int v, int off, const Ctx& c) {
+    static CF_HD void conv_middle(R& run, Sh& sh, int v, int off, const Ctx& c, int X = -1, int k = 0) {
+        const bool carry = kDefer && X >= 0;
+        // slice k was loaded at the end of the caller's first phase (after its stage input)
         for (int r = 1; r < P::NTW; ++r)
             run.phase([&](Thread& th, int t) {
                 if (r == P::NTW - 1) fetch_hat(th, t, c);
                 E::fwdr(r, th, sh, t, *c.d, c.m);
+                if (carry && r == 1) defer_store(th, sh, t, X, k, c);
             });
-        run.phase([&](Thread& th, int t) { E::final_round(false, th, sh, t, *c.d, c.m, c.a.hat); });
+        run.phase([&](Thread& th, int t) {
+            E::final_round(false, th, sh, t, *c.d, c.m, c.a.hat);
+            if (carry) defer_load(th, t, X, k + 1, c);
+        });
         for (int r = P::NTW - 1; r >= 1; --r)
             run.phase([&](Thread& th, int t) {
                 if (r == 1) fetch_ifact(th, t, v, off, c);
                 E::invr(r, th, sh, t, *c.d, c.m);
+                if (carry && r == P::NTW - 1) defer_store(th, sh, t, X, k + 1, c);
             });
     }
 
@@ -409,9 +492,14 @@ struct Task {
             const bool s1 = v > c.v0;   // rows v <= v0 of V are zero, so column v of S is zero
             const bool s2 = v >= c.v0;  // g(v) = 0 below v0, so column v of V is zero
             const double g = s2 ? g_of(v, c) : 0.0;
-            if ((v >> 4) != segY && (s1 || s2)) {
+            const int Xd = kDefer ? (v >> 4) - 1 : -1;  // block whose deferred tiles move this step
+            const int kd = 4 * (v & 15);                 // its slices kd .. kd+3 belong to this step
+            if ((v >> 4) != segY && (s1 || s2 || Xd >= 0)) {
                 segY = v >> 4;
-                run.phase([&](Thread&, int t) { load_segrow(sh, t, segY, c); });
+                run.phase([&](Thread&, int t) {
+                    load_segrow(sh, t, segY, c);
+                    if (Xd >= 0) defer_list(sh, t, Xd, c);
+                });
             }
             if (s1) {
                 const int nact = active_queries(v, c);
@@ -427,6 +515,7 @@ struct Task {
                             return cnt0;
                         }, c.m);
                         E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
+                        if (kDefer && Xd >= 0) defer_load(th, t, Xd, kd, c);
                     });
                 } else {
                     run.phase([&](Thread& th, int t) {
@@ -434,9 +523,12 @@ struct Task {
                         keep_row_out(th, t, v, c);
                     });
                     queries(run, sh, v, 0, c);
-                    run.phase([&](Thread& th, int t) { E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0); });
+                    run.phase([&](Thread& th, int t) {
+                        E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
+                        if (kDefer && Xd >= 0) defer_load(th, t, Xd, kd, c);
+                    });
                 }
-                conv_middle(run, sh, v, 0, c);
+                conv_middle(run, sh, v, 0, c, Xd, kd);
                 // last inverse round, stage-1 output (then row v of S2 is fetched), query finish,
                 // power tables of stage 2
                 run.phase([&](Thread& th, int t) {
@@ -450,6 +542,7 @@ struct Task {
                     zero_col(c.a.Sst, t, v, v, nm, c);
                     if (s2) powtab(sh, t, v, g, c);
                 });
+                if (Xd >= 0) defer_phase(run, sh, Xd, kd, kd + 2, c);
             }
             // fetch ahead for step v + 1 unless a flush (and a new row block) comes in between
             const bool next = s2 && (v & 15) != 15;
@@ -460,8 +553,9 @@ struct Task {
                         const double bpart = stage2_pre(th, sh, t, v, c);
                         run.wave_partial1(sh.red2, th, t, bpart, c.m);
                         E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
+                        if (kDefer && Xd >= 0) defer_load(th, t, Xd, kd + 2, c);
                     });
-                    conv_middle(run, sh, v, 1, c);
+                    conv_middle(run, sh, v, 1, c, Xd, kd + 2);
                     run.phase([&](Thread& th, int t) {
                         double B = 0.0;
                         for (int w = 0; w < NW; ++w) B += sh.red2[w];  // NW <= 16 values below p/2 + 4
@@ -475,20 +569,26 @@ struct Task {
                         stage2_post(th, sh, t, v, false, 0.0, c);
                         if (next) fetch_next(th, sh, t, v, c);
                     });
+                    if (Xd >= 0) defer_phase(run, sh, Xd, kd + 2, kd + 4, c);
                 }
             } else {
                 run.phase([&](Thread&, int t) {
                     zero_col(c.a.Vst, t, v, v + 1, nm + 1, c);
                     if (t == 0) sh.scal[1] = 0.0;  // the hand-over entry (v+1, v) is zero too
                 });
+                if (Xd >= 0) defer_phase(run, sh, Xd, kd + 2, kd + 4, c);
             }
             pref = next;
             if ((v & 15) == 15) {
-                run.phase([&](Thread&, int t) { flush_tiles(sh, t, v >> 4, c); });
-                run.phase([&](Thread&, int t) {
-                    flush(sh, c.a.Vpool, c.a.Vst, t, v >> 4, c);
-                    flush(sh, c.a.Spool, c.a.Sst, t, v >> 4, c);
-                });
+                if (kDefer) {
+                    run.phase([&](Thread&, int t) { flush_head(t, v >> 4, c); });
+                } else {
+                    run.phase([&](Thread&, int t) { flush_tiles(sh, t, v >> 4, c); });
+                    run.phase([&](Thread&, int t) {
+                        flush(sh, c.a.Vpool, c.a.Vst, t, v >> 4, c);
+                        flush(sh, c.a.Spool, c.a.Sst, t, v >> 4, c);
+                    });
+                }
             }
         }
         // row nm of V only feeds the queries with n = nm
