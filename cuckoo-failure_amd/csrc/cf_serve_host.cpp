@@ -29,7 +29,8 @@ using namespace cf;
 using namespace cf_int;
 
 struct cf_tables {
-    cf_engine* e = nullptr;
+    cf_engine* e = nullptr;  // must outlive the set (queries report through it)
+    int device = 0;
     int d = 0, model = 0, nmax = 0, Mmax = 0, F1 = 0, v0 = 0;
     std::vector<uint64_t> primes;
     double* store = nullptr;  // np x stride doubles
@@ -227,6 +228,7 @@ int cf_tables_build(cf_engine* e, int d, int model, int nmax, int Mmax, const ui
     HIP_TRY(e, hipSetDevice(e->device), "hipSetDevice");
     auto t = std::make_unique<cf_tables>();
     t->e = e;
+    t->device = e->device;
     t->d = d;
     t->model = model;
     t->nmax = nmax;
@@ -285,7 +287,7 @@ int cf_tables_build(cf_engine* e, int d, int model, int nmax, int Mmax, const ui
 
 void cf_tables_free(cf_tables* t) {
     if (!t) return;
-    (void)hipSetDevice(t->e->device);
+    (void)hipSetDevice(t->device);
     if (t->stream) (void)hipStreamSynchronize(t->stream);
     if (t->store) (void)hipFree(t->store);
     if (t->dprimes) (void)hipFree(t->dprimes);
@@ -400,7 +402,7 @@ int cf_server_open(cf_tables* t, int slots, int max_batch, int crt_threads, cf_s
 
 void cf_server_close(cf_server* s) {
     if (!s) return;
-    (void)hipSetDevice(s->t->e->device);
+    (void)hipSetDevice(s->t->device);
     for (Slot& sl : s->slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         free_slot(s, sl);

@@ -99,6 +99,8 @@ typedef struct cf_tables cf_tables;
  * memory (CF_E_NOMEM otherwise). Requires Mmax > nmax >= 1 and Mmax < every prime < 2^50. */
 int cf_tables_build(cf_engine* e, int d, int model, int nmax, int Mmax, const uint64_t* primes, size_t np,
                     cf_tables** out);
+/* Frees a set. Servers on it must be closed first; its engine must still be open while the set
+ * is queried (errors and stats are reported through the engine). */
 void cf_tables_free(cf_tables* t);
 /* Device bytes of the resident store. */
 size_t cf_tables_bytes(const cf_tables* t);

@@ -77,14 +77,21 @@ def test_fast_resume_scenarios(idx):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("idx", range(0, 80, 7))
+@pytest.mark.parametrize("idx", range(0, len(load("fast_cases.json")), 7))
 def test_fast_cli_matches_reference_lines(idx):
-    recs = load("fast_cases.json")
-    if idx >= len(recs):
-        pytest.skip("fewer cases")
-    rec = recs[idx]
+    rec = load("fast_cases.json")[idx]
     rc, so, se, _ = run_fast(rec["d"], rec["queries"], rec["first"], rec["count"])
     assert rc == rec["rc"] == 0, se
+    assert so == rec["stdout"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("idx", [1, 3])
+def test_fast_cli_tables_beyond_16384_rows(idx):
+    # the reference's own lines for n = 16385 and for two queries of 20000 / 17000 rows
+    rec = load("fast_wide.json")[idx]
+    rc, so, se, _ = run_fast(rec["d"], rec["queries"], rec["prime"], 1)
+    assert rc == 0, se
     assert so == rec["stdout"]
 
 
