@@ -41,6 +41,11 @@ class Instance(ctypes.Structure):
                 ("prime_off", ctypes.c_int64), ("prime_cnt", ctypes.c_int64)]
 
 
+class Exact(ctypes.Structure):
+    _fields_ = [("log2_pfail", ctypes.c_char * 48), ("le", ctypes.c_int32 * 3), ("p_fail", ctypes.c_char * 64),
+                ("nbad_bits", ctypes.c_int64), ("qn_bits", ctypes.c_int64), ("error", ctypes.c_char * 128)]
+
+
 class ServeStats(ctypes.Structure):
     _fields_ = [("wall_s", ctypes.c_double), ("queries_per_s", ctypes.c_double), ("p50_ms", ctypes.c_double),
                 ("p99_ms", ctypes.c_double), ("mean_batch", ctypes.c_double), ("batches", ctypes.c_long)]
@@ -80,6 +85,10 @@ def lib():
                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(u64)]
         L.cf_crt_log2.argtypes = [ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.c_size_t]
         L.cf_crt_log2.restype = ctypes.c_double
+        L.cf_exact_combine.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                       ctypes.c_size_t, ctypes.POINTER(Exact), ctypes.c_char_p, ctypes.c_size_t]
+        L.cf_post_need.argtypes = [ctypes.c_int] * 4
+        L.cf_post_need.restype = ctypes.c_long
         L.cf_server_open.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
         L.cf_server_close.argtypes = [vp]
         L.cf_server_close.restype = None
@@ -132,6 +141,29 @@ def crt_log2(primes, res):
     """log2 of the CRT combination of res modulo primes (host Garner); -inf for 0."""
     k = len(primes)
     return lib().cf_crt_log2((u64 * max(k, 1))(*primes), (u64 * max(k, 1))(*res), k)
+
+
+def exact_combine(d, n, M, primes, res, decimal=False):
+    """The exact columns of one post_gmp.py row for N_bad = CRT(res mod primes), Q = C(M, d):
+    a dict with log2_pfail, le (three bools), p_fail, nbad_bits, qn_bits (and nbad, the decimal
+    digits, with decimal=True)."""
+    k = len(primes)
+    out = Exact()
+    buf = ctypes.create_string_buffer(20 * k + 2) if decimal else None
+    rc = lib().cf_exact_combine(d, n, M, (u64 * max(k, 1))(*primes), (u64 * max(k, 1))(*res), k, ctypes.byref(out),
+                                buf, 20 * k + 2 if decimal else 0)
+    if rc:
+        raise CfError(rc, out.error.decode())
+    r = {"log2_pfail": out.log2_pfail.decode(), "le": [bool(x) for x in out.le], "p_fail": out.p_fail.decode(),
+         "nbad_bits": out.nbad_bits, "qn_bits": out.qn_bits}
+    if decimal:
+        r["nbad"] = buf.value.decode()
+    return r
+
+
+def post_need(n, M, d, pbits=31):
+    """post_gmp.py's need(n, M) (double precision)."""
+    return lib().cf_post_need(n, M, d, pbits)
 
 
 def mem_per_prime(nmax, Mmax, J=1):

@@ -117,6 +117,26 @@ int cf_query(cf_tables* t, const int32_t* n, const int32_t* M, size_t J, const i
  * as fast/post_gmp.py's CRT merge, in long double from the leading digits); -inf for x = 0. */
 double cf_crt_log2(const uint64_t* primes, const uint64_t* res, size_t k);
 
+/* ---- the exact combiner: fast/post_gmp.py's columns without GMP (host only) ---- */
+typedef struct {
+    char log2_pfail[48];  /* log2(N_bad / Q^n) correctly rounded to 15 decimals; "-inf" for N_bad = 0 */
+    int32_t le[3];        /* 1 iff N_bad * 2^lambda <= Q^n, lambda = 40, 64, 128 (exact) */
+    char p_fail[64];      /* N_bad / Q^n truncated to 40 significant digits ("d.ddd...e-X"), or "0" */
+    int64_t nbad_bits;    /* bit lengths of N_bad and of Q^n = C(M, d)^n */
+    int64_t qn_bits;
+    char error[128];      /* why a call failed */
+} cf_exact;
+/* One row of post_gmp.py (fast/post_gmp.py:51-60 crt, :63-76 sci, :79-91 the row): N_bad is the
+ * x in [0, prod primes) with x = res[i] mod primes[i] (odd, pairwise coprime moduli < 2^63), and
+ * Q = C(M, d). CF_E_PRIMES when prod primes <= 4 Q^n, CF_E_ARGS when x > Q^n or the moduli are
+ * unusable (post_gmp.py:84's assertion and gmpy2.invert's error). With nbad_dec, also writes
+ * N_bad in decimal (dec_len >= 20 k + 2 always suffices). */
+int cf_exact_combine(int d, int n, int M, const uint64_t* primes, const uint64_t* res, size_t k, cf_exact* out,
+                     char* nbad_dec, size_t dec_len);
+/* post_gmp.py's need(n, M) in double precision (fast/post_gmp.py:46-48): how many primes above
+ * 2^pbits its CRT uses. It can differ from cf_need(), the engine's long-double need. -1 for M < d. */
+long cf_post_need(int n, int M, int d, int pbits);
+
 /* The serving loop over one resident set. Requests are admitted into up to `slots` batches in
  * flight at once (each on its own stream; fewer if device memory runs short), a free slot taking
  * every request waiting at that moment (at most max_batch); finished requests are combined by CRT
