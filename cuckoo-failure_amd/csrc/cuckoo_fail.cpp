@@ -1,13 +1,14 @@
 // cuckoo_fail -- the reference's 32-bit command line (src/cuckoo_fail.cpp) for the one-table
 // model, computed on the MI355X engine:
 //
-//   cuckoo_fail single -n N -M M -d D [--replacement] [--algo ntt] [--exact] [--mod P]
+//   cuckoo_fail single -n N -M M -d D [--replacement] [--algo forward|cubic|ntt] [--exact] [--mod P]
 //
 // The prime set (32-bit NTT-friendly primes, largest first, enough for Q^n), the printed lines
 // and their formats are those of the reference, so the output is identical apart from the time
-// fields. Every residue is computed on the GPU (one workgroup per prime); the host only chooses
-// the primes and reconstructs N_bad. The separate-table model and the forward / cubic algorithms
-// are not part of this engine and are rejected with exit status 2.
+// fields. Every residue is computed on the GPU; the host only chooses the primes and reconstructs
+// N_bad. --algo ntt (default) runs the suffix-table engine, --algo cubic and --algo forward the
+// reference's O(n^3) and O(n^4) algorithms as device kernels (cf_run_algo). The separate-table
+// model is not part of this engine and is rejected with exit status 2.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -35,8 +36,7 @@ void usage() {
         "  --exact    also print N_bad and Q^n in decimal\n"
         "  --mod P    report N_bad mod P only (single run, no reconstruction)\n"
         "\n"
-        "This build runs `single --algo ntt` on the GPU; `separate` and --algo forward|cubic are\n"
-        "not available.");
+        "This build runs `single` on the GPU (all three algorithms); `separate` is not available.");
 }
 
 struct Options {
@@ -78,7 +78,6 @@ Options parse(int argc, char** argv) {
     if (o.mode == "separate") throw std::invalid_argument("separate tables are not available in this build");
     if (o.algo.empty()) o.algo = "ntt";
     if (o.algo != "forward" && o.algo != "cubic" && o.algo != "ntt") throw std::invalid_argument("bad --algo");
-    if (o.algo != "ntt") throw std::invalid_argument("--algo " + o.algo + " is not available in this build (ntt only)");
     if (o.d < 1 || o.n < 1 || o.M <= o.n) throw std::invalid_argument("one table: need d >= 1 and 1 <= n < M");
     if (o.mod && (o.mod < 3 || o.mod >= (1ll << 32) || !is_prime32(o.mod))) throw std::invalid_argument("--mod must be a prime below 2^32");
     return o;
@@ -155,7 +154,11 @@ int main(int argc, char** argv) {
     const cfcli::Clock clock;
     std::vector<uint64_t> p64(primes.begin(), primes.end()), res(primes.size());
     const int32_t qn = o.n, qM = o.M;
-    const int rc = cf_run_fused(e, o.d, o.replacement ? 1 : 0, &qn, &qM, 1, p64.data(), p64.size(), nullptr, res.data());
+    const int model = o.replacement ? 1 : 0;
+    const int rc = o.algo == "ntt"
+                       ? cf_run_fused(e, o.d, model, &qn, &qM, 1, p64.data(), p64.size(), nullptr, res.data())
+                       : cf_run_algo(e, o.algo == "cubic" ? CF_ALGO_CUBIC : CF_ALGO_FORWARD, o.d, model, o.n, o.M,
+                                     p64.data(), p64.size(), res.data());
     const double secs = clock.secs();
     if (rc != CF_OK) {
         std::fprintf(stderr, "error: %s: %s\n", cf_strerror(rc), cf_last_error(e));

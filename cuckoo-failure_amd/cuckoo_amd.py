@@ -19,6 +19,7 @@ CLI_FAST = os.path.join(HERE, "bin", "cuckoo_fast")
 CLI_FAIL = os.path.join(HERE, "bin", "cuckoo_fail")
 
 CF_OK, CF_E_DEVICE, CF_E_ARGS, CF_E_NOMEM, CF_E_PRIMES, CF_E_UNSUPPORTED = 0, 1, 2, 3, 4, 5
+CF_ALGO_FORWARD, CF_ALGO_CUBIC = 1, 2
 
 _lib = None
 u64 = ctypes.c_uint64
@@ -72,6 +73,8 @@ def lib():
                                    ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(u64)]
         L.cf_run_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Instance), ctypes.c_size_t, ctypes.POINTER(u64),
                                    ctypes.POINTER(u64)]
+        L.cf_run_algo.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ctypes.POINTER(u64), ctypes.c_size_t,
+                                                                           ctypes.POINTER(u64)]
         vp = ctypes.c_void_p
         L.cf_tables_build.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(u64),
                                       ctypes.c_size_t, ctypes.POINTER(vp)]
@@ -200,6 +203,18 @@ class Engine:
         buf = ctypes.create_string_buffer(256)
         lib().cf_device_name(self._h, buf, 256)
         return buf.value.decode()
+
+    def run_algo(self, algo, n, M, primes, d, model=0):
+        """N_bad(n, M) mod p for every prime by the device forward (algo "forward", O(n^4)) or cubic
+        (algo "cubic", O(n^3)) algorithm."""
+        code = {"forward": CF_ALGO_FORWARD, "cubic": CF_ALGO_CUBIC}[algo]
+        np_ = len(primes)
+        P = (u64 * max(np_, 1))(*primes)
+        res = (u64 * max(np_, 1))()
+        rc = lib().cf_run_algo(self._h, code, d, model, n, M, P, np_, res)
+        if rc != CF_OK:
+            raise CfError(rc, lib().cf_last_error(self._h).decode())
+        return list(res)[:np_]
 
     def run(self, queries, primes, d, model=0, mask=None):
         """N_bad(n, M) mod p for every prime and query: returns res[b][j] (0 where mask[b][j] is 0)."""

@@ -89,6 +89,18 @@ typedef struct {
  * indices; indices no instance covers are left untouched. */
 int cf_run_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t* primes, uint64_t* res);
 
+/* The reference's direct one-table algorithms on the device, one workgroup per prime, as
+ * cross-checks of the suffix-table path: res[b] = N_bad(n, M) mod primes[b] for model 0 (d-subsets)
+ * or 1 (with replacement). CF_ALGO_FORWARD is one_table_forward, the O(n^4) boundary recurrence
+ * (include/cuckoo/single_table.hpp:44-78), n <= 256; CF_ALGO_CUBIC is
+ * build_one_table_suffix_cubic + OneTableSuffix::nbad (single_table.hpp:135-166, 104-120),
+ * n <= 1024 (CF_E_UNSUPPORTED beyond). Selected by `cuckoo_fail single --algo forward|cubic`
+ * (src/cuckoo_fail.cpp:60-87). Primes odd, M < p < 2^50. */
+#define CF_ALGO_FORWARD 1
+#define CF_ALGO_CUBIC 2
+int cf_run_algo(cf_engine* e, int algo, int d, int model, int n, int M, const uint64_t* primes, size_t np,
+                uint64_t* res);
+
 /* ---- resident tables and serving ---- */
 typedef struct cf_tables cf_tables;
 

@@ -12,6 +12,8 @@
  *     (tests/golden/check_ref.txt), the d = 1 closed form M^n - M!/(M-n)!, ntt = cubic, and the
  *     reference's 32-bit CLI output (tests/golden/fail_*.txt);
  *   - or_ntt_primes32: the prime lists implied by the reference CLI's product/primes lines;
+ *   - or_nbad_forward: the reference CLI's --algo forward outputs (tests/golden/fail_algo_cases.json),
+ *     the brute-force counts, and forward = cubic = ntt;
  *   - or_brute_force: the reference test program's brute-force counts;
  *   - or_cyclic_conv_*: each other (schoolbook vs NTT) on random inputs.
  */
@@ -313,6 +315,57 @@ int or_suffix_table_cubic(int nm, int d, int rep, u64 p, u64* V) {
         }
     }
     free(S), free(gp);
+    fact_free(&F);
+    return 0;
+}
+
+/* ---- Algorithm 1, the O(n^4) forward recurrence over block boundaries
+ *      (single_table.hpp:25-31 phi, :44-78 one_table_forward), with the full M x (M+1) table ---- */
+
+static u64 phi_ab(const Fact* F, long long a, long long b, long long r, u64 p) {
+    if (b <= 0) return binom(F, a - 1, r, p);
+    const u64 t = mulm(binom(F, a - b - 1, r - b, p), binom(F, r - 1, b - 1, p), p);
+    return t ? p - t : 0;
+}
+
+int or_nbad_forward(int d, int rep, u64 p, int n, int M, u64* out) {
+    if (n < 1 || M <= n || d < 1) return -1;
+    Fact F;
+    if (fact_init(&F, M, p)) return -1;
+    const int z = M - n;
+    const size_t HB = (size_t)(n + 1), FB = (size_t)(M + 1);
+    u64* hpow = (u64*)malloc(sizeof(u64) * FB * HB); /* hpow[b][u] = g(M - b)^u */
+    u64* Ft = (u64*)calloc((size_t)M * FB, sizeof(u64)); /* F[l][b] */
+    if (!hpow || !Ft) return -1;
+    for (int b = 0; b <= M; ++b) {
+        const u64 h = model_g(&F, M - b, d, rep, p);
+        hpow[(size_t)b * HB] = 1 % p;
+        for (int u = 1; u <= n; ++u) hpow[(size_t)b * HB + u] = mulm(hpow[(size_t)b * HB + u - 1], h, p);
+    }
+#define NU(l) ((l) > z ? (l) - z : 0)
+#define TAU(l, b, r, c)                                                                                     \
+    mulm(mulm(mulm(phi_ab(&F, (c) - (l), (b) - (l), (r) - (l), p), binom(&F, M - (b), (c) - (b), p), p),    \
+              binom(&F, n - NU(l), (r) - ((l) > z ? (l) : z), p), p),                                        \
+         hpow[(size_t)(b) * HB + ((r) - ((l) > z ? (l) : z))], p)
+    Ft[0] = 1 % p;
+    for (int r = z; r < M; ++r)
+        for (int c = r + 1; c <= M; ++c) {
+            u64 s = mulm(Ft[0], TAU(0, 0, r, c), p);
+            for (int l = z; l < r; ++l)
+                for (int b = l + 1; b < c; ++b)
+                    if (Ft[(size_t)l * FB + b]) s = addm(s, mulm(Ft[(size_t)l * FB + b], TAU(l, b, r, c), p), p);
+            Ft[(size_t)r * FB + c] = s;
+        }
+    u64 total = 0;
+    for (int l = 0; l < M; ++l)
+        for (int b = 0; b <= M; ++b)
+            if (Ft[(size_t)l * FB + b])
+                total = addm(total, mulm(mulm(Ft[(size_t)l * FB + b], binom(&F, M - l - 1, M - b, p), p),
+                                         hpow[(size_t)b * HB + n - NU(l)], p), p);
+#undef TAU
+#undef NU
+    *out = total;
+    free(hpow), free(Ft);
     fact_free(&F);
     return 0;
 }

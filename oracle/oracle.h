@@ -45,6 +45,9 @@ int or_suffix_table_cubic(int nm, int d, int rep, uint64_t p, uint64_t* V);
 /* N_bad mod p for each query from a table built at nm = max n, evaluated with the paper's double
  * sum (single_table.hpp:103-119). algo 0: ntt table, 1: cubic table. */
 int or_nbad(int algo, int d, int rep, uint64_t p, int nq, const int* n, const int* M, uint64_t* out);
+/* N_bad(n, M) mod p by Algorithm 1, the O(n^4) forward recurrence over block boundaries
+ * (include/cuckoo/single_table.hpp:25-31, 44-78). */
+int or_nbad_forward(int d, int rep, uint64_t p, int n, int M, uint64_t* out);
 /* N_bad mod p evaluated with the fast engine's query rewrite and Horner finish
  * (fast/cuckoo_fast.cu:306-323, 877-890) on the ntt table. */
 int or_nbad_horner(int d, int rep, uint64_t p, int nq, const int* n, const int* M, uint64_t* out);

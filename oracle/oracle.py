@@ -44,6 +44,7 @@ def lib():
                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(u64)]
         L.or_nbad_horner.argtypes = [ctypes.c_int, ctypes.c_int, u64, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(u64)]
+        L.or_nbad_forward.argtypes = [ctypes.c_int, ctypes.c_int, u64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(u64)]
         L.or_brute_force.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_ulonglong)] * 2
         for f in (L.or_cyclic_conv_schoolbook, L.or_cyclic_conv_ntt):
             f.argtypes = [u64, ctypes.c_size_t, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)]
@@ -91,13 +92,19 @@ def suffix_table(nm, d, p, rep=False, algo="ntt"):
 
 
 def nbad(d, p, queries, rep=False, algo="ntt"):
-    """N_bad mod p for each (n, M) by the paper's double sum (algo 'ntt', 'cubic') or by the fast
-    engine's Horner form (algo 'horner')."""
+    """N_bad mod p for each (n, M) by the paper's double sum (algo 'ntt', 'cubic'), by the fast
+    engine's Horner form (algo 'horner') or by the O(n^4) forward recurrence (algo 'forward')."""
     ns = _arr(ctypes.c_int, [q[0] for q in queries])
     Ms = _arr(ctypes.c_int, [q[1] for q in queries])
     out = (u64 * len(queries))()
     if algo == "horner":
         rc = lib().or_nbad_horner(d, int(rep), p, len(queries), ns, Ms, out)
+    elif algo == "forward":
+        rc = 0
+        for j, (n, M) in enumerate(queries):
+            x = u64()
+            rc = rc or lib().or_nbad_forward(d, int(rep), p, n, M, ctypes.byref(x))
+            out[j] = x.value
     else:
         rc = lib().or_nbad(0 if algo == "ntt" else 1, d, int(rep), p, len(queries), ns, Ms, out)
     if rc:

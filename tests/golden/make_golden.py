@@ -9,6 +9,7 @@ the programs' outputs are kept, as data.
     python tests/golden/make_golden.py --big      # also n = 65536 (about 32 GB RAM, ~3 min)
     python tests/golden/make_golden.py --only uneven   # just the uneven-batch fixture
     python tests/golden/make_golden.py --only wide     # just the n > 16384 fixture (~2 min, 8 cores)
+    python tests/golden/make_golden.py --only algos    # just the --algo forward|cubic fixture
 """
 import json
 import os
@@ -170,6 +171,38 @@ def fail_cases():
     return recs
 
 
+def fail_algo_cases():
+    """The reference's 32-bit CLI with its O(n^4) forward and O(n^3) cubic algorithms."""
+    args = []
+    for algo in ("forward", "cubic"):
+        args += [
+            ["single", "-n", "20", "-M", "40", "-d", "3", "--algo", algo, "--exact"],
+            ["single", "-n", "6", "-M", "7", "-d", "2", "--algo", algo, "--exact"],
+            ["single", "-n", "1", "-M", "2", "-d", "1", "--algo", algo, "--exact"],
+            ["single", "-n", "12", "-M", "20", "-d", "1", "--algo", algo, "--exact"],
+            ["single", "-n", "31", "-M", "32", "-d", "2", "--algo", algo, "--exact"],
+            ["single", "-n", "60", "-M", "100", "-d", "3", "--algo", algo, "--exact"],
+            ["single", "-n", "64", "-M", "65", "-d", "3", "--algo", algo],
+            ["single", "-n", "30", "-M", "50", "-d", "2", "--replacement", "--algo", algo, "--exact"],
+            ["single", "-n", "25", "-M", "40", "-d", "3", "--replacement", "--algo", algo, "--exact"],
+            ["single", "-n", "40", "-M", "41", "-d", "5", "--algo", algo, "--mod", "998244353"],
+        ]
+    args += [
+        ["single", "-n", "200", "-M", "400", "-d", "3", "--algo", "cubic", "--exact"],
+        ["single", "-n", "100", "-M", "150", "-d", "4", "--algo", "cubic"],
+        ["single", "-n", "128", "-M", "1000", "-d", "2", "--replacement", "--algo", "cubic", "--mod", "1004535809"],
+        ["single", "-n", "80", "-M", "90", "-d", "4", "--algo", "forward"],
+    ]
+    with ThreadPoolExecutor(max_workers=os.cpu_count()) as ex:
+        outs = list(ex.map(lambda a: run([FAIL, *a]), args))
+    recs = []
+    for a, (rc, so, se) in zip(args, outs):
+        so = re.sub(r"time: [0-9.]+ s", "time: T s", so)
+        so = re.sub(r"\([0-9.]+ s\)", "(T s)", so)
+        recs.append({"args": a, "rc": rc, "stdout": so, "stderr": se})
+    return recs
+
+
 def uneven_golden(n_max=512):
     """The uneven-batch instance list (seed 0) and, for its instances with n <= n_max, the
     reference's residues over the instance's whole prime set (one cuckoo_fast run per instance,
@@ -209,6 +242,10 @@ def main():
         with open(os.path.join(HERE, "uneven_batch.json"), "w") as f:
             json.dump(uneven_golden(), f, indent=0)
         return
+    if only == "algos":
+        with open(os.path.join(HERE, "fail_algo_cases.json"), "w") as f:
+            json.dump(fail_algo_cases(), f, indent=0)
+        return
     if only == "wide":
         with open(os.path.join(HERE, "fast_wide.json"), "w") as f:
             json.dump(fast_wide(), f, indent=0)
@@ -225,6 +262,8 @@ def main():
         json.dump(fast_resume(), f, indent=0)
     with open(os.path.join(HERE, "fail_cases.json"), "w") as f:
         json.dump(fail_cases(), f, indent=0)
+    with open(os.path.join(HERE, "fail_algo_cases.json"), "w") as f:
+        json.dump(fail_algo_cases(), f, indent=0)
     with open(os.path.join(HERE, "fast_16384.json"), "w") as f:
         json.dump(fast_big(16384, 32768, 0, 8), f, indent=0)
     with open(os.path.join(HERE, "uneven_batch.json"), "w") as f:

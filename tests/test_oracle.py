@@ -204,3 +204,40 @@ def test_uneven_batch_instances_pinned():
     for x in g["golden"]:
         assert tuple(g["instances"][x["index"]]) == (x["n"], x["d"], x["M"])
         assert oracle.need(x["n"], x["M"], x["d"]) == x["need"]
+
+
+def test_forward_oracle_against_reference_forward_runs():
+    """The reference CLI's --algo forward --exact N_bad, reduced mod 50-bit primes, equals the
+    oracle's O(n^4) recurrence; --mod runs give the residue itself."""
+    primes = oracle.prime_range(0, 3)
+    seen = 0
+    for rec in load("fail_algo_cases.json"):
+        a = rec["args"]
+        if "forward" not in a:
+            continue
+        n, M, d = int(a[a.index("-n") + 1]), int(a[a.index("-M") + 1]), int(a[a.index("-d") + 1])
+        rep = "--replacement" in a
+        nbad = fail_field(rec["stdout"], "N_bad")
+        if "--mod" in a:
+            P = int(a[a.index("--mod") + 1])
+            want = int(rec["stdout"].split(f"N_bad mod {P} = ")[1].split()[0])
+            assert oracle.nbad(d, P, [(n, M)], rep=rep, algo="forward")[0] == want
+            seen += 1
+        elif nbad is not None:
+            for p in primes:
+                assert oracle.nbad(d, p, [(n, M)], rep=rep, algo="forward")[0] == int(nbad) % p, a
+            seen += 1
+    assert seen >= 8
+
+
+def test_forward_oracle_against_brute_force_and_ntt():
+    for n, M, d, rep, nbad, _ in check_instances():
+        for p in P_CHECK:
+            assert oracle.nbad(d, p, [(n, M)], rep=rep, algo="forward")[0] == nbad % p, (n, M, d, rep)
+    rng = random.Random(4)
+    for _ in range(25):
+        n = rng.randint(1, 25)
+        M = n + rng.randint(1, 20)
+        d, rep = rng.randint(1, 5), bool(rng.randint(0, 1))
+        p = rng.choice(P_CHECK)
+        assert oracle.nbad(d, p, [(n, M)], rep=rep, algo="forward") == oracle.nbad(d, p, [(n, M)], rep=rep)
