@@ -73,7 +73,7 @@ def lib():
                                    ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(u64)]
         L.cf_run_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Instance), ctypes.c_size_t, ctypes.POINTER(u64),
                                    ctypes.POINTER(u64)]
-        L.cf_run_algo.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ctypes.POINTER(u64), ctypes.c_size_t,
+        L.cf_run_algo.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 5 + [ctypes.POINTER(u64), ctypes.c_size_t,
                                                                            ctypes.POINTER(u64)]
         vp = ctypes.c_void_p
         L.cf_tables_build.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(u64),
