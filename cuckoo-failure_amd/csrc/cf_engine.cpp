@@ -198,6 +198,124 @@ int timed_launch(cf_engine* e, L launch, const std::vector<QueryDesc>& qd, const
     return CF_OK;
 }
 
+// ---- graph mode: the call's launches recorded, then run as one captured hipGraph ----
+struct Rec {
+    int kind;  // 0: k_tasks<lg>, 1: the work-queue kernel
+    int lg, grid, njobs;
+    int64_t stride;  // queue: arena doubles per workgroup
+    size_t o_jb, o_td, o_qd, o_sl, o_nx;
+};
+
+struct Recorder {
+    std::vector<char> stage;  // host image of every launch's descriptors, 256-byte aligned regions
+    std::vector<Rec> recs;
+    std::vector<QueryDesc> outs;
+    size_t put(const void* p, size_t n) {
+        const size_t o = (stage.size() + 255) & ~(size_t)255;
+        stage.resize(o + std::max<size_t>(n, 1), 0);
+        if (n) memcpy(stage.data() + o, p, n);
+        return o;
+    }
+    void add(int kind, int lg, int grid, int64_t stride, const std::vector<TaskDesc>& td, const std::vector<QueryDesc>& qd,
+             const std::vector<int>& slots, const std::vector<QueueJob>* jobs) {
+        static const unsigned zero = 0;  // the queue position, reset by every replay's upload
+        Rec r{kind, lg, grid, jobs ? (int)jobs->size() : 0, stride, 0, 0, 0, 0, 0};
+        if (jobs) r.o_jb = put(jobs->data(), sizeof(QueueJob) * jobs->size());
+        r.o_td = put(td.data(), sizeof(TaskDesc) * td.size());
+        r.o_qd = put(qd.data(), sizeof(QueryDesc) * qd.size());
+        r.o_sl = put(slots.data(), sizeof(int) * slots.size());
+        r.o_nx = put(&zero, sizeof zero);
+        recs.push_back(r);
+        outs.insert(outs.end(), qd.begin(), qd.end());
+    }
+};
+
+void drop_graph(cf_engine* e) {
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->graph_exec = nullptr;
+    e->graph = nullptr;
+    e->graph_sig.clear();
+}
+
+int ensure_pinned(cf_engine* e, void** buf, size_t* cap, size_t bytes) {
+    if (bytes <= *cap) return CF_OK;
+    drop_graph(e);  // the graph holds the old address
+    if (*buf) (void)hipHostFree(*buf);
+    *buf = nullptr, *cap = 0;
+    HIP_TRY(e, hipHostMalloc(buf, bytes, hipHostMallocDefault), "hipHostMalloc");
+    *cap = bytes;
+    return CF_OK;
+}
+
+// The captured chain: staging upload -> every launch in order -> result download.
+int capture(cf_engine* e, const Recorder& R, size_t o_res, size_t nres) {
+    HIP_TRY(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    char* sm = (char*)e->small;
+    hipError_t st = hipMemcpyAsync(sm, e->pin_stage, R.stage.size(), hipMemcpyHostToDevice, e->stream);
+    for (size_t k = 0; st == hipSuccess && k < R.recs.size(); ++k) {
+        const Rec& r = R.recs[k];
+        uint64_t* res = (uint64_t*)(sm + o_res);
+        if (r.kind == 1)
+            st = launch_queue(r.grid, r.njobs, (const QueueJob*)(sm + r.o_jb), (const TaskDesc*)(sm + r.o_td),
+                              (const QueryDesc*)(sm + r.o_qd), (const int*)(sm + r.o_sl), (double*)e->arena, r.stride, res,
+                              (unsigned*)(sm + r.o_nx), e->stream);
+        else
+            st = launch_tasks(r.lg, r.grid, (const TaskDesc*)(sm + r.o_td), (const QueryDesc*)(sm + r.o_qd),
+                              (const int*)(sm + r.o_sl), (double*)e->arena, res, e->stream);
+    }
+    if (st == hipSuccess)
+        st = hipMemcpyAsync(e->pin_res, sm + o_res, sizeof(uint64_t) * std::max<size_t>(nres, 1), hipMemcpyDeviceToHost,
+                            e->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t st_end = hipStreamEndCapture(e->stream, &g);
+    if (st != hipSuccess || st_end != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(e, st != hipSuccess ? st : st_end, "graph capture");
+    }
+    hipGraphExec_t x = nullptr;
+    st = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (st != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, st, "hipGraphInstantiate");
+    }
+    e->graph = g;
+    e->graph_exec = x;
+    return CF_OK;
+}
+
+// Runs the recorded launches as one graph (captured now, or replayed when the signature of the
+// plan and of the buffers it points into is the one already captured).
+int run_graph(cf_engine* e, const Recorder& R, size_t nres, uint64_t* res, float& ms) {
+    const size_t o_res = (R.stage.size() + 255) & ~(size_t)255;
+    int rc = ensure(e, &e->small, &e->small_cap, o_res + sizeof(uint64_t) * std::max<size_t>(nres, 1), "the task descriptors");
+    if (rc) return rc;
+    if ((rc = ensure_pinned(e, (void**)&e->pin_stage, &e->pin_stage_cap, R.stage.size()))) return rc;
+    if ((rc = ensure_pinned(e, (void**)&e->pin_res, &e->pin_res_cap, sizeof(uint64_t) * std::max<size_t>(nres, 1))))
+        return rc;
+    std::vector<int64_t> sig = {(int64_t)R.stage.size(), (int64_t)nres, (int64_t)(uintptr_t)e->small,
+                                (int64_t)(uintptr_t)e->arena, (int64_t)(uintptr_t)e->pin_stage, (int64_t)(uintptr_t)e->pin_res};
+    for (const Rec& r : R.recs)
+        sig.insert(sig.end(), {r.kind, r.lg, r.grid, r.njobs, r.stride, (int64_t)r.o_jb, (int64_t)r.o_td, (int64_t)r.o_qd,
+                               (int64_t)r.o_sl, (int64_t)r.o_nx});
+    memcpy(e->pin_stage, R.stage.data(), R.stage.size());
+    if (sig != e->graph_sig || !e->graph_exec) {
+        drop_graph(e);
+        if ((rc = capture(e, R, o_res, nres))) return rc;
+        e->graph_sig = sig;
+        e->stats.graph = 1;
+    } else {
+        e->stats.graph = 2;
+    }
+    HIP_TRY(e, hipEventRecord(e->ev0, e->stream), "hipEventRecord");
+    HIP_TRY(e, hipGraphLaunch(e->graph_exec, e->stream), "hipGraphLaunch");
+    HIP_TRY(e, hipEventRecord(e->ev1, e->stream), "hipEventRecord");
+    HIP_TRY(e, hipStreamSynchronize(e->stream), "graph execution");
+    HIP_TRY(e, hipEventElapsedTime(&ms, e->ev0, e->ev1), "hipEventElapsedTime");
+    for (const QueryDesc& q : R.outs) res[q.out] = e->pin_res[q.out];
+    return CF_OK;
+}
+
 // XCD-local group barriers for the 2^16 kernel, unless CF_WIDE_DEVICE_BARRIERS=1 asks for
 // device-scope ones everywhere.
 int wide_local() {
@@ -211,6 +329,10 @@ int wide_local() {
 int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* res, bool queue) {
     const size_t budget = usable_bytes(e);
     float kernel_ms_total = 0;
+    // graph mode records the launches and runs them as one graph at the end (not with the
+    // cooperative 2^16 launches)
+    const bool record = e->graphs && std::none_of(tasks.begin(), tasks.end(), [](const TaskPlan& t) { return t.lg == kWideLog2; });
+    Recorder R;
     std::vector<TaskDesc> td;
     std::vector<QueryDesc> qd;
     std::vector<int> slots;
@@ -246,16 +368,20 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
             int rc = ensure(e, &e->arena, &e->arena_cap, wg * stride, "the task tables");
             if (rc) return rc;
             describe(e, tasks, 0, nq, true, td, qd, slots);
-            Uploaded u;
-            rc = upload(e, td, qd, slots, nres, u, &jobs);
-            if (rc) return rc;
-            float ms = 0;
-            rc = timed_launch(e, [&] {
-                return launch_queue((int)wg, (int)jobs.size(), u.jobs, u.td, u.qd, u.slots, (double*)e->arena,
-                                    (int64_t)(stride / 8), u.res, u.next, e->stream);
-            }, qd, u, nres, res, ms);
-            if (rc) return rc;
-            kernel_ms_total += ms;
+            if (record) {
+                R.add(1, tasks[0].lg, (int)wg, (int64_t)(stride / 8), td, qd, slots, &jobs);
+            } else {
+                Uploaded u;
+                rc = upload(e, td, qd, slots, nres, u, &jobs);
+                if (rc) return rc;
+                float ms = 0;
+                rc = timed_launch(e, [&] {
+                    return launch_queue((int)wg, (int)jobs.size(), u.jobs, u.td, u.qd, u.slots, (double*)e->arena,
+                                        (int64_t)(stride / 8), u.res, u.next, e->stream);
+                }, qd, u, nres, res, ms);
+                if (rc) return rc;
+                kernel_ms_total += ms;
+            }
             e->stats.launches++;
             e->stats.tasks += (int)nq;
             e->stats.log2_length = std::max(e->stats.log2_length, tasks[0].lg);
@@ -298,6 +424,15 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
         describe(e, tasks, at, end, false, td, qd, slots);
         int rc = ensure(e, &e->arena, &e->arena_cap, bytes, "the task tables");
         if (rc) return rc;
+        e->stats.launches++;
+        e->stats.tasks += nt;
+        e->stats.log2_length = std::max(e->stats.log2_length, lg);
+        e->stats.device_bytes = std::max(e->stats.device_bytes, bytes);
+        if (record) {
+            R.add(0, lg, nt, 0, td, qd, slots, nullptr);
+            at = end;
+            continue;
+        }
         Uploaded u;
         rc = upload(e, td, qd, slots, nres, u, nullptr, wide ? wide_sync_words(nt) : 0);
         if (rc) return rc;
@@ -313,11 +448,11 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
             if (aborted) return fail(e, CF_E_DEVICE, "a group barrier of the 2^16 transform timed out");
         }
         kernel_ms_total += ms;
-        e->stats.launches++;
-        e->stats.tasks += nt;
-        e->stats.log2_length = std::max(e->stats.log2_length, lg);
-        e->stats.device_bytes = std::max(e->stats.device_bytes, bytes);
         at = end;
+    }
+    if (record && !R.recs.empty()) {
+        const int rc = run_graph(e, R, nres, res, kernel_ms_total);
+        if (rc) return rc;
     }
     e->stats.batches = e->stats.launches;
     e->stats.kernel_ms = kernel_ms_total;
@@ -366,10 +501,20 @@ int cf_open(int device, cf_engine** out) {
     return CF_OK;
 }
 
+int cf_set_graphs(cf_engine* e, int on) {
+    if (!e) return CF_E_ARGS;
+    e->graphs = on != 0;
+    if (!e->graphs) drop_graph(e);
+    return CF_OK;
+}
+
 void cf_close(cf_engine* e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    if (e->pin_stage) (void)hipHostFree(e->pin_stage);
+    if (e->pin_res) (void)hipHostFree(e->pin_res);
     if (e->arena) (void)hipFree(e->arena);
     if (e->small) (void)hipFree(e->small);
     if (e->ev0) (void)hipEventDestroy(e->ev0);

@@ -23,6 +23,16 @@ struct cf_engine {
     void* small = nullptr;  // tasks, queries, slot tables, results
     size_t small_cap = 0;
     std::map<int, cf::SlotPlan> plans;  // by nm
+    // graph mode (cf_set_graphs): a call's uploads, launches and downloads are one hipGraph,
+    // captured once per launch-plan signature and replayed while the signature repeats
+    bool graphs = false;
+    std::vector<int64_t> graph_sig;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    char* pin_stage = nullptr;  // pinned host staging of the descriptors
+    size_t pin_stage_cap = 0;
+    uint64_t* pin_res = nullptr;  // pinned host copy of the results
+    size_t pin_res_cap = 0;
 };
 
 namespace cf_int {

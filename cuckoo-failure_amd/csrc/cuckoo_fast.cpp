@@ -122,6 +122,9 @@ int main(int argc, char** argv) {
     // ---- engine and batch size ----
     cf_engine* e = cfcli::open_engine();
     if (!e) return 1;
+    // batches of one shape (all but the last) replay one captured graph; CF_GRAPHS=0 turns it off
+    const char* graphs = std::getenv("CF_GRAPHS");
+    cf_set_graphs(e, !(graphs && graphs[0] == '0'));
     const size_t J = qs.size();
     const size_t per_prime = cf_mem_per_prime(nmax, Mmax, J);
     if (batch <= 0) batch = cf_max_batch(e, nmax, Mmax, J);

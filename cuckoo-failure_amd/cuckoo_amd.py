@@ -34,7 +34,7 @@ class CfError(RuntimeError):
 class Stats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double), ("launches", ctypes.c_int),
                 ("tasks", ctypes.c_int), ("batches", ctypes.c_int), ("log2_length", ctypes.c_int),
-                ("device_bytes", ctypes.c_size_t)]
+                ("device_bytes", ctypes.c_size_t), ("graph", ctypes.c_int)]
 
 
 class Instance(ctypes.Structure):
@@ -68,6 +68,7 @@ def lib():
         L.cf_last_error.argtypes = [ctypes.c_void_p]
         L.cf_last_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
         L.cf_device_name.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+        L.cf_set_graphs.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.cf_run_fused.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
                                    ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t, ctypes.POINTER(u64), ctypes.c_size_t,
                                    ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(u64)]
@@ -203,6 +204,11 @@ class Engine:
         buf = ctypes.create_string_buffer(256)
         lib().cf_device_name(self._h, buf, 256)
         return buf.value.decode()
+
+    def set_graphs(self, on=True):
+        """Graph mode: each run / run_batch call is one captured hipGraph, replayed while the launch
+        plan repeats."""
+        lib().cf_set_graphs(self._h, 1 if on else 0)
 
     def run_algo(self, algo, n, M, primes, d, model=0):
         """N_bad(n, M) mod p for every prime by the device forward (algo "forward", O(n^4)) or cubic

@@ -42,6 +42,7 @@ typedef struct {
     int batches;
     int log2_length; /* log2 of the longest transform of the call */
     size_t device_bytes;
+    int graph;       /* 0: launched one by one; 1: captured into a hipGraph and run; 2: graph replayed */
 } cf_stats;
 
 /* Replaces the CUDA back end's device set-up (fast/cuckoo_fast.cu:328-359): selects a HIP device
@@ -54,6 +55,14 @@ const char* cf_last_error(const cf_engine* e);
 int cf_last_stats(const cf_engine* e, cf_stats* out);
 /* Device name and architecture, e.g. "AMD Instinct MI355X gfx950". */
 int cf_device_name(const cf_engine* e, char* buf, size_t len);
+/* Graph mode for cf_run_fused / cf_run_batch (off by default). On, a call's descriptor upload, its
+ * kernel launches and the result download are captured into one hipGraph (no memset nodes; the
+ * queue counter is reset by the upload) and the graph is replayed by later calls whose launch
+ * plan has the same signature (same shapes and counts, new primes), with only the pinned staging
+ * rewritten: the reference's per-batch host-precompute -> build -> query -> finish chain
+ * (fast/cuckoo_fast.cu:787-898) as one submission. Calls with 2^16 transforms (cooperative
+ * launches) always run one launch at a time. */
+int cf_set_graphs(cf_engine* e, int on);
 
 /* Builds the suffix table for each prime and answers the queries on it: the reference's per-batch
  * work of fast/cuckoo_fast.cu:787-896 (host precompute, kernel spectrum, the v-loop of stage-1 /
