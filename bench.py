@@ -220,6 +220,8 @@ def main():
     ap.add_argument("--slots", type=int, default=2)
     ap.add_argument("--max-batch", type=int, default=32)
     ap.add_argument("--crt-threads", type=int, default=8)
+    ap.add_argument("--autotune", action="store_true",
+                    help="fast: time the 2^14 kernel variants on the workload's own tables first and run the fastest")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (primes shard across GPUs by "
@@ -235,6 +237,7 @@ def main():
     import cuckoo_amd as cf
     eng = cf.Engine()
     primes = cf.prime_range(0, a.primes)
+    tune = eng.autotune(14, n=N) if a.autotune else None
     for _ in range(a.warmup):
         eng.run([(N, M)], primes, D)
     kernel_ms, launches, tasks = 0.0, 0, 0
@@ -277,6 +280,11 @@ def main():
                      "modmul_per_launch": mm_per_launch},
         "cpu_baseline": base,
     }
+    if tune:
+        out["autotune"] = tune
+        if tune["chosen"]:  # the profiled traffic is the built-in variant's
+            out["roofline"]["kernel"] = f"k_tasks<14> variant {tune['chosen']}"
+            out["roofline"]["traffic"] = None
     print(json.dumps(out))
 
 

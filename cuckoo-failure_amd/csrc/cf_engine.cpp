@@ -201,7 +201,7 @@ int timed_launch(cf_engine* e, L launch, const std::vector<QueryDesc>& qd, const
 // ---- graph mode: the call's launches recorded, then run as one captured hipGraph ----
 struct Rec {
     int kind;  // 0: k_tasks<lg>, 1: the work-queue kernel
-    int lg, grid, njobs;
+    int lg, grid, njobs, variant;
     int64_t stride;  // queue: arena doubles per workgroup
     size_t o_jb, o_td, o_qd, o_sl, o_nx;
 };
@@ -216,10 +216,10 @@ struct Recorder {
         if (n) memcpy(stage.data() + o, p, n);
         return o;
     }
-    void add(int kind, int lg, int grid, int64_t stride, const std::vector<TaskDesc>& td, const std::vector<QueryDesc>& qd,
-             const std::vector<int>& slots, const std::vector<QueueJob>* jobs) {
+    void add(int kind, int lg, int grid, int variant, int64_t stride, const std::vector<TaskDesc>& td,
+             const std::vector<QueryDesc>& qd, const std::vector<int>& slots, const std::vector<QueueJob>* jobs) {
         static const unsigned zero = 0;  // the queue position, reset by every replay's upload
-        Rec r{kind, lg, grid, jobs ? (int)jobs->size() : 0, stride, 0, 0, 0, 0, 0};
+        Rec r{kind, lg, grid, jobs ? (int)jobs->size() : 0, variant, stride, 0, 0, 0, 0, 0};
         if (jobs) r.o_jb = put(jobs->data(), sizeof(QueueJob) * jobs->size());
         r.o_td = put(td.data(), sizeof(TaskDesc) * td.size());
         r.o_qd = put(qd.data(), sizeof(QueryDesc) * qd.size());
@@ -262,7 +262,7 @@ int capture(cf_engine* e, const Recorder& R, size_t o_res, size_t nres) {
                               (unsigned*)(sm + r.o_nx), e->stream);
         else
             st = launch_tasks(r.lg, r.grid, (const TaskDesc*)(sm + r.o_td), (const QueryDesc*)(sm + r.o_qd),
-                              (const int*)(sm + r.o_sl), (double*)e->arena, res, e->stream);
+                              (const int*)(sm + r.o_sl), (double*)e->arena, res, e->stream, r.variant);
     }
     if (st == hipSuccess)
         st = hipMemcpyAsync(e->pin_res, sm + o_res, sizeof(uint64_t) * std::max<size_t>(nres, 1), hipMemcpyDeviceToHost,
@@ -296,7 +296,7 @@ int run_graph(cf_engine* e, const Recorder& R, size_t nres, uint64_t* res, float
     std::vector<int64_t> sig = {(int64_t)R.stage.size(), (int64_t)nres, (int64_t)(uintptr_t)e->small,
                                 (int64_t)(uintptr_t)e->arena, (int64_t)(uintptr_t)e->pin_stage, (int64_t)(uintptr_t)e->pin_res};
     for (const Rec& r : R.recs)
-        sig.insert(sig.end(), {r.kind, r.lg, r.grid, r.njobs, r.stride, (int64_t)r.o_jb, (int64_t)r.o_td, (int64_t)r.o_qd,
+        sig.insert(sig.end(), {r.kind, r.lg, r.grid, r.njobs, r.variant, r.stride, (int64_t)r.o_jb, (int64_t)r.o_td, (int64_t)r.o_qd,
                                (int64_t)r.o_sl, (int64_t)r.o_nx});
     memcpy(e->pin_stage, R.stage.data(), R.stage.size());
     if (sig != e->graph_sig || !e->graph_exec) {
@@ -369,7 +369,7 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
             if (rc) return rc;
             describe(e, tasks, 0, nq, true, td, qd, slots);
             if (record) {
-                R.add(1, tasks[0].lg, (int)wg, (int64_t)(stride / 8), td, qd, slots, &jobs);
+                R.add(1, tasks[0].lg, (int)wg, 0, (int64_t)(stride / 8), td, qd, slots, &jobs);
             } else {
                 Uploaded u;
                 rc = upload(e, td, qd, slots, nres, u, &jobs);
@@ -407,8 +407,9 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
         size_t cls_end = end;
         while (cls_end < tasks.size() && tasks[cls_end].lg == lg) ++cls_end;
         const bool wide = lg == kWideLog2;
+        const int variant = wide ? 0 : e->variant[lg];
         const size_t cap = wide ? (size_t)std::min(kWideMaxTasks, std::max(wide_resident(e->device), 0) / 16)
-                                : (size_t)std::max(resident_tasks(lg, e->device), 1);
+                                : (size_t)std::max(resident_tasks(lg, e->device, variant), 1);
         if (!cap) return fail(e, CF_E_UNSUPPORTED, "the device cannot hold the 16 workgroups of a 2^16 transform");
         const size_t fit = std::min(end - at, wide ? cap : end - at), left = cls_end - at;
         size_t take = fit >= cap ? fit / cap * cap : fit;
@@ -429,7 +430,7 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
         e->stats.log2_length = std::max(e->stats.log2_length, lg);
         e->stats.device_bytes = std::max(e->stats.device_bytes, bytes);
         if (record) {
-            R.add(0, lg, nt, 0, td, qd, slots, nullptr);
+            R.add(0, lg, nt, variant, 0, td, qd, slots, nullptr);
             at = end;
             continue;
         }
@@ -439,7 +440,7 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
         float ms = 0;
         rc = timed_launch(e, [&] {
             if (wide) return launch_wide(nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, u.sync, wide_local(), e->stream);
-            return launch_tasks(lg, nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, e->stream);
+            return launch_tasks(lg, nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, e->stream, variant);
         }, qd, u, nres, res, ms);
         if (rc) return rc;
         if (wide) {
@@ -614,6 +615,77 @@ int cf_run_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t
     const int rc = execute(e, tasks, (size_t)nres, res, true);
     e->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return rc;
+}
+
+int cf_set_variant(cf_engine* e, int log2_length, int variant) {
+    if (!e || log2_length < kMinLog2L || log2_length > kMaxLog2L) return CF_E_ARGS;
+    if (variant < 0) variant = 0;
+    if (variant >= variant_count(log2_length)) return CF_E_ARGS;
+    e->variant[log2_length] = variant;
+    return CF_OK;
+}
+
+int cf_autotune(cf_engine* e, int log2_length, int n_cal, cf_tune* out) {
+    if (!e || !out || log2_length < kMinLog2L || log2_length > kMaxLog2L) return CF_E_ARGS;
+    const int L = 1 << log2_length;
+    if (n_cal && (n_cal > L || (log2_length > kMinLog2L && n_cal <= L / 2) || n_cal < 1)) return CF_E_ARGS;
+    memset(out, 0, sizeof *out);
+    e->err.clear();
+    HIP_TRY(e, hipSetDevice(e->device), "hipSetDevice");
+    const int lg = log2_length, nv = std::min(variant_count(lg), 4);
+    // calibration: tables of n rows (the caller's, else the class's smallest, L/2 + 1), M = 2n,
+    // d = 3, two waves of the most resident variant's workgroups (every variant at full occupancy)
+    const int n = n_cal ? n_cal : L / 2 + 1, M = 2 * n;
+    int most = 1;
+    for (int v = 0; v < nv; ++v) most = std::max(most, resident_tasks(lg, e->device, v));
+    const size_t per = task_bytes(plan_for(e, n), M + 1, 1, lg);
+    const size_t count = std::max<size_t>(1, std::min<size_t>(2 * (size_t)most, usable_bytes(e) / per));
+    const std::vector<uint64_t> primes = prime_range(50, 0, (long)count);
+    const int saved = e->variant[lg];
+    std::vector<uint64_t> first;
+    auto run = [&](int v, size_t np, std::vector<uint64_t>& res) -> int {
+        e->variant[lg] = v;
+        res.assign(np, 0);
+        std::vector<TaskPlan> tasks(np);
+        for (size_t b = 0; b < np; ++b) {
+            int rc = plan_task(e, primes[b], 3, 0, 1, [&](size_t) { return QueryDesc{n, M, (int64_t)b}; }, tasks[b]);
+            if (rc) return rc;
+        }
+        e->stats = cf_stats{};
+        const bool g = e->graphs;
+        e->graphs = false;
+        const int rc = execute(e, tasks, np, res.data(), false);
+        e->graphs = g;
+        return rc;
+    };
+    for (int v = 0; v < nv; ++v) {
+        std::vector<uint64_t> res;
+        int rc = run(v, 1, res);  // code load and first-touch of this variant
+        if (!rc) rc = run(v, count, res);
+        if (rc) {
+            e->variant[lg] = saved;
+            return rc;
+        }
+        const KernelVariant kv = variant_info(lg, v);
+        out->waves[v] = kv.waves;
+        out->deferred[v] = kv.deferred;
+        out->resident[v] = resident_tasks(lg, e->device, v);
+        out->ms[v] = e->stats.kernel_ms;
+        if (v == 0) first = res;
+        else if (res != first) {
+            e->variant[lg] = saved;
+            return fail(e, CF_E_DEVICE, "kernel variant " + std::to_string(v) + " of 2^" + std::to_string(lg) +
+                                            " gave residues different from variant 0");
+        }
+        if (out->ms[v] < out->ms[out->chosen]) out->chosen = v;
+    }
+    out->log2_length = lg;
+    out->variants = nv;
+    out->tables = (int)count;
+    out->n = n;
+    e->variant[lg] = out->chosen;
+    e->stats = cf_stats{};
+    return CF_OK;
 }
 
 // ---- host logic, no device ----

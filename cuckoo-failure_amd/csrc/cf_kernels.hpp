@@ -8,12 +8,26 @@ namespace cf {
 
 constexpr int kMinLog2L = 10, kMaxLog2L = 14;
 
-// Workgroups of the log2L kernel resident on the device at once (0 if unknown).
-int resident_tasks(int log2L, int device);
+// Compiled variants of the task kernel per transform length (variant 0 the built-in choice):
+// waves per transform (virtual threads per hardware thread) and whether the tile flush of a
+// column block is deferred into the next block's transform rounds.
+struct KernelVariant {
+    int waves;
+    int deferred;
+};
+int variant_count(int log2L);
+KernelVariant variant_info(int log2L, int variant);
+
+// Workgroups of the log2L kernel (variant) resident on the device at once (0 if unknown).
+int resident_tasks(int log2L, int device, int variant = 0);
 
 // One workgroup per task; log2L selects the kernel instance (transform length 2^log2L).
 hipError_t launch_tasks(int log2L, int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem,
-                        uint64_t* res, hipStream_t stream);
+                        uint64_t* res, hipStream_t stream, int variant = 0);
+// The variants other than 0 (cf_variants.hip).
+int resident_variant(int log2L, int variant, int device);
+hipError_t launch_variant(int log2L, int variant, int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots,
+                          double* mem, uint64_t* res, hipStream_t stream);
 
 // The work-queue kernel: jobs of tasks with log2 L in [kMinLog2L, kQueueMaxLog2L], any mix,
 // ordered as they should start. A job is up to kQueueSlots(lg) consecutive tasks of identical

@@ -34,7 +34,9 @@ struct Ctx {
     int es, eb, L;
 };
 
-template <int LOG2L, int CAP = LOG2L>
+// DEFER: -1 the built-in choice for the length (deferred tile flush where the transform has the
+// carrier rounds), 0 never; a task-kernel variant the autotuner can pick.
+template <int LOG2L, int CAP = LOG2L, int DEFER = -1>
 struct Task {
     using P = Plan<LOG2L>;
     using E = Engine<LOG2L>;
@@ -315,7 +317,7 @@ struct Task {
     // each convolution, whose th.pa is free (loads issued one phase earlier, after the stage
     // input or the pointwise product has used th.pa; stores at the end of the round);
     // the staging is double-buffered by block parity, so block X+1 writes the other half.
-    static constexpr bool kDefer = P::NTW >= 3 && CAP == LOG2L;
+    static constexpr bool kDefer = DEFER != 0 && P::NTW >= 3 && CAP == LOG2L;
     static CF_HD void flush_head(int t, int X, const Ctx& c) {
         if (X + 1 >= c.d->NB) return;
         const int NS = c.d->NS, cb = (X & 1) << 4;

@@ -126,6 +126,15 @@ int main(int argc, char** argv) {
     const char* graphs = std::getenv("CF_GRAPHS");
     cf_set_graphs(e, !(graphs && graphs[0] == '0'));
     const size_t J = qs.size();
+    // CF_AUTOTUNE=1: time the kernel variants of this run's transform length on its largest n
+    const char* tune = std::getenv("CF_AUTOTUNE");
+    if (tune && tune[0] == '1') {
+        const int lg = cf_engine_log2_length(nmax, 1 + (uint64_t(1) << 49));
+        cf_tune t;
+        if (lg >= 10 && lg <= 14 && cf_autotune(e, lg, nmax > (1 << (lg - 1)) ? nmax : 0, &t) == CF_OK)
+            std::fprintf(stderr, "autotune 2^%d on n=%d: variant %d of %d (%.1f ms)\n", lg, t.n, t.chosen, t.variants,
+                         t.ms[t.chosen]);
+    }
     const size_t per_prime = cf_mem_per_prime(nmax, Mmax, J);
     if (batch <= 0) batch = cf_max_batch(e, nmax, Mmax, J);
     if (batch < 1) {

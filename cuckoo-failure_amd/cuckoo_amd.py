@@ -47,6 +47,12 @@ class Exact(ctypes.Structure):
                 ("nbad_bits", ctypes.c_int64), ("qn_bits", ctypes.c_int64), ("error", ctypes.c_char * 128)]
 
 
+class Tune(ctypes.Structure):
+    _fields_ = [("log2_length", ctypes.c_int), ("variants", ctypes.c_int), ("chosen", ctypes.c_int),
+                ("tables", ctypes.c_int), ("n", ctypes.c_int), ("waves", ctypes.c_int * 4), ("deferred", ctypes.c_int * 4),
+                ("resident", ctypes.c_int * 4), ("ms", ctypes.c_double * 4)]
+
+
 class ServeStats(ctypes.Structure):
     _fields_ = [("wall_s", ctypes.c_double), ("queries_per_s", ctypes.c_double), ("p50_ms", ctypes.c_double),
                 ("p99_ms", ctypes.c_double), ("mean_batch", ctypes.c_double), ("batches", ctypes.c_long)]
@@ -69,6 +75,8 @@ def lib():
         L.cf_last_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
         L.cf_device_name.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
         L.cf_set_graphs.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.cf_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tune)]
+        L.cf_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.cf_run_fused.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
                                    ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t, ctypes.POINTER(u64), ctypes.c_size_t,
                                    ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(u64)]
@@ -209,6 +217,23 @@ class Engine:
         """Graph mode: each run / run_batch call is one captured hipGraph, replayed while the launch
         plan repeats."""
         lib().cf_set_graphs(self._h, 1 if on else 0)
+
+    def autotune(self, log2_length, n=0):
+        """Times every compiled kernel variant of one transform-length class on tables of n rows
+        (0: the class's smallest) and keeps the fastest for this engine; returns the table."""
+        t = Tune()
+        rc = lib().cf_autotune(self._h, log2_length, n, ctypes.byref(t))
+        if rc != CF_OK:
+            raise CfError(rc, lib().cf_last_error(self._h).decode())
+        k = t.variants
+        return {"log2_length": t.log2_length, "chosen": t.chosen, "tables": t.tables, "n": t.n,
+                "variants": [{"waves_per_transform": t.waves[v], "deferred_flush": bool(t.deferred[v]),
+                              "resident_workgroups": t.resident[v], "kernel_ms": round(t.ms[v], 2)} for v in range(k)]}
+
+    def set_variant(self, log2_length, variant):
+        rc = lib().cf_set_variant(self._h, log2_length, variant)
+        if rc != CF_OK:
+            raise CfError(rc, "bad variant")
 
     def run_algo(self, algo, n, M, primes, d, model=0):
         """N_bad(n, M) mod p for every prime by the device forward (algo "forward", O(n^4)) or cubic

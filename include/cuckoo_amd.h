@@ -110,6 +110,31 @@ int cf_run_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t
 int cf_run_algo(cf_engine* e, int algo, int d, int model, int n, int M, const uint64_t* primes, size_t np,
                 uint64_t* res);
 
+/* The autotuner the batched forms use (no counterpart in the reference, whose CUDA build has one
+ * fixed schedule per length, fast/cuckoo_fast.cu:510-646). Each transform-length class 2^10 ..
+ * 2^14 has compiled variants of the per-prime kernel that differ in waves per transform (one or
+ * two virtual threads per hardware thread) and in the tile flush (immediate, or deferred into the
+ * next column block's rounds); variant 0 is the built-in choice. cf_autotune times every variant
+ * of one class on a calibration batch (tables of n rows, M = 2n, d = 3, two waves of workgroups;
+ * n = n_cal, which must lie in the class (L/2, L], or L/2 + 1 for n_cal = 0), checks that all give
+ * identical residues, and makes the fastest the engine's choice for that class in later calls.
+ * The best variant can depend on n within a class: tune on the sizes that will run. The
+ * work-queue kernel of cf_run_batch is not affected. */
+typedef struct {
+    int log2_length;  /* the class */
+    int variants;     /* how many variants were timed */
+    int chosen;       /* the fastest, now used by this engine */
+    int tables;       /* calibration tables per variant */
+    int n;            /* their rows */
+    int waves[4];     /* waves per transform */
+    int deferred[4];  /* 1: deferred tile flush */
+    int resident[4];  /* workgroups resident on the device at once */
+    double ms[4];     /* kernel time of the calibration batch */
+} cf_tune;
+int cf_autotune(cf_engine* e, int log2_length, int n_cal, cf_tune* out);
+/* Selects a class's variant by hand (0 = the built-in choice). */
+int cf_set_variant(cf_engine* e, int log2_length, int variant);
+
 /* ---- resident tables and serving ---- */
 typedef struct cf_tables cf_tables;
 
