@@ -27,15 +27,18 @@ struct DevRunner {
     int t;  // hardware thread
     // The thread index is passed through an empty asm statement, so the index arithmetic of a
     // phase is recomputed there instead of being hoisted out of the v-loop and kept live (in
-    // spilled registers) across all steps.
+    // spilled registers) across all steps. All virtual threads' indices are formed first, so the
+    // scheduler can start the second one's loads under the first one's arithmetic.
     template <class F>
     __device__ __forceinline__ void phase(F f) {
+        int tt[VT];
 #pragma unroll
         for (int i = 0; i < VT; ++i) {
-            int tt = t + i * TP;
-            asm volatile("" : "+v"(tt));
-            f(th[i], tt);
+            tt[i] = t + i * TP;
+            asm volatile("" : "+v"(tt[i]));
         }
+#pragma unroll
+        for (int i = 0; i < VT; ++i) f(th[i], tt[i]);
         __syncthreads();
     }
     // Inside a phase: this wave's sums of the cnt partials, written by lane 0 to red[wave][8].
