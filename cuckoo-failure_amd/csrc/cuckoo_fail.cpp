@@ -1,14 +1,15 @@
-// cuckoo_fail -- the reference's 32-bit command line (src/cuckoo_fail.cpp) for the one-table
-// model, computed on the MI355X engine:
+// cuckoo_fail -- the reference's 32-bit command line (src/cuckoo_fail.cpp), computed on the
+// MI355X engine:
 //
-//   cuckoo_fail single -n N -M M -d D [--replacement] [--algo forward|cubic|ntt] [--exact] [--mod P]
+//   cuckoo_fail single   -n N -M M -d D [--replacement] [--algo forward|cubic|ntt] [--exact] [--mod P]
+//   cuckoo_fail separate -n N -m m1,m2,...,md [--algo forward|suffix] [--exact] [--mod P]
 //
 // The prime set (32-bit NTT-friendly primes, largest first, enough for Q^n), the printed lines
 // and their formats are those of the reference, so the output is identical apart from the time
 // fields. Every residue is computed on the GPU; the host only chooses the primes and reconstructs
 // N_bad. --algo ntt (default) runs the suffix-table engine, --algo cubic and --algo forward the
-// reference's O(n^3) and O(n^4) algorithms as device kernels (cf_run_algo). The separate-table
-// model is not part of this engine and is rejected with exit status 2.
+// reference's O(n^3) and O(n^4) algorithms as device kernels (cf_run_algo); `separate` runs the
+// separate-tables model's suffix or forward algorithm as device kernels (cf_run_separate).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -36,15 +37,28 @@ void usage() {
         "  --exact    also print N_bad and Q^n in decimal\n"
         "  --mod P    report N_bad mod P only (single run, no reconstruction)\n"
         "\n"
-        "This build runs `single` on the GPU (all three algorithms); `separate` is not available.");
+        "This build runs every algorithm of both modes on the GPU.");
 }
 
 struct Options {
     std::string mode, algo;
     int n = 0, M = 0, d = 0;
+    std::vector<int> m;
     bool replacement = false, exact = false;
     long long mod = 0;
 };
+
+// "a,b,c" -> {a, b, c}, each piece read as atoi reads it (src/cuckoo_fail.cpp:41-51)
+std::vector<int> parse_list(const std::string& s) {
+    std::vector<int> out;
+    for (size_t pos = 0; pos <= s.size();) {
+        size_t comma = s.find(',', pos);
+        if (comma == std::string::npos) comma = s.size();
+        out.push_back(std::atoi(s.substr(pos, comma - pos).c_str()));
+        pos = comma + 1;
+    }
+    return out;
+}
 
 bool is_prime32(long long x) {
     if (x < 2) return false;
@@ -53,7 +67,7 @@ bool is_prime32(long long x) {
     return true;
 }
 
-// Argument handling of src/cuckoo_fail.cpp:54-87 for the one-table mode.
+// Argument handling of src/cuckoo_fail.cpp:54-87.
 Options parse(int argc, char** argv) {
     if (argc < 2) throw std::invalid_argument("missing mode");
     Options o;
@@ -68,17 +82,29 @@ Options parse(int argc, char** argv) {
         if (a == "-n") o.n = std::atoi(next().c_str());
         else if (a == "-M") o.M = std::atoi(next().c_str());
         else if (a == "-d") o.d = std::atoi(next().c_str());
-        else if (a == "-m") (void)next();
+        else if (a == "-m") o.m = parse_list(next());
         else if (a == "--algo") o.algo = next();
         else if (a == "--mod") o.mod = std::atoll(next().c_str());
         else if (a == "--replacement") o.replacement = true;
         else if (a == "--exact") o.exact = true;
         else throw std::invalid_argument("unknown option " + a);
     }
-    if (o.mode == "separate") throw std::invalid_argument("separate tables are not available in this build");
-    if (o.algo.empty()) o.algo = "ntt";
-    if (o.algo != "forward" && o.algo != "cubic" && o.algo != "ntt") throw std::invalid_argument("bad --algo");
-    if (o.d < 1 || o.n < 1 || o.M <= o.n) throw std::invalid_argument("one table: need d >= 1 and 1 <= n < M");
+    if (o.mode == "single") {
+        if (o.algo.empty()) o.algo = "ntt";
+        if (o.algo != "forward" && o.algo != "cubic" && o.algo != "ntt") throw std::invalid_argument("bad --algo");
+        if (o.d < 1 || o.n < 1 || o.M <= o.n) throw std::invalid_argument("one table: need d >= 1 and 1 <= n < M");
+    } else {
+        if (o.algo.empty()) o.algo = "suffix";
+        if (o.algo != "forward" && o.algo != "suffix") throw std::invalid_argument("bad --algo");
+        // separate_tables.hpp:17-25
+        if (o.m.empty()) throw std::invalid_argument("separate tables: need d >= 1 tables");
+        long long M = 0;
+        for (int x : o.m) {
+            if (x < 1) throw std::invalid_argument("separate tables: lengths must be positive");
+            M += x;
+        }
+        if (o.n < 1 || M <= o.n) throw std::invalid_argument("separate tables: need 1 <= n < M");
+    }
     if (o.mod && (o.mod < 3 || o.mod >= (1ll << 32) || !is_prime32(o.mod))) throw std::invalid_argument("--mod must be a prime below 2^32");
     return o;
 }
@@ -86,7 +112,9 @@ Options parse(int argc, char** argv) {
 // log2 Q as src/cuckoo_fail.cpp:101-111 (long double).
 long double log2_Q(const Options& o) {
     long double r = 0;
-    if (o.replacement) r = o.d * std::log2(static_cast<long double>(o.M));
+    if (o.mode == "separate") {
+        for (int x : o.m) r += std::log2(static_cast<long double>(x));
+    } else if (o.replacement) r = o.d * std::log2(static_cast<long double>(o.M));
     else
         for (int i = 0; i < o.d; ++i) r += std::log2(static_cast<long double>(o.M - i)) - std::log2(static_cast<long double>(i + 1));
     return r;
@@ -96,7 +124,10 @@ long double log2_Q(const Options& o) {
 cfcli::Decimal exact_Qn(const Options& o) {
     cfcli::Decimal b;
     b.d = {1};
-    if (o.replacement) {
+    if (o.mode == "separate") {
+        for (int x : o.m)
+            for (int t = 0; t < o.n; ++t) b.mul_add(x, 0);
+    } else if (o.replacement) {
         for (int i = 0; i < o.d * o.n; ++i) b.mul_add(o.M, 0);
     } else {
         for (int i = 0; i < o.d; ++i)
@@ -118,8 +149,14 @@ int main(int argc, char** argv) {
         usage();
         return 2;
     }
-    std::printf("one table: n=%d items, M=%d slots, d=%d choices, %s\n", o.n, o.M, o.d,
-                o.replacement ? "d independent draws with replacement" : "uniform d-subset of candidate slots");
+    if (o.mode == "single") {
+        std::printf("one table: n=%d items, M=%d slots, d=%d choices, %s\n", o.n, o.M, o.d,
+                    o.replacement ? "d independent draws with replacement" : "uniform d-subset of candidate slots");
+    } else {
+        std::printf("separate tables: n=%d items, m=(", o.n);
+        for (size_t j = 0; j < o.m.size(); ++j) std::printf("%s%d", j ? "," : "", o.m[j]);
+        std::printf("), d=%zu tables\n", o.m.size());
+    }
     const long double bits = o.n * log2_Q(o);
     std::printf("algorithm: %s;  outcomes Q^n = 2^%.3f\n", o.algo.c_str(), static_cast<double>(bits));
 
@@ -155,7 +192,11 @@ int main(int argc, char** argv) {
     std::vector<uint64_t> p64(primes.begin(), primes.end()), res(primes.size());
     const int32_t qn = o.n, qM = o.M;
     const int model = o.replacement ? 1 : 0;
-    const int rc = o.algo == "ntt"
+    const std::vector<int32_t> m32(o.m.begin(), o.m.end());
+    const int rc = o.mode == "separate"
+                       ? cf_run_separate(e, o.algo == "suffix" ? CF_ALGO_SUFFIX : CF_ALGO_FORWARD, o.n, m32.data(),
+                                         (int)m32.size(), p64.data(), p64.size(), res.data())
+                   : o.algo == "ntt"
                        ? cf_run_fused(e, o.d, model, &qn, &qM, 1, p64.data(), p64.size(), nullptr, res.data())
                        : cf_run_algo(e, o.algo == "cubic" ? CF_ALGO_CUBIC : CF_ALGO_FORWARD, o.d, model, o.n, o.M,
                                      p64.data(), p64.size(), res.data());

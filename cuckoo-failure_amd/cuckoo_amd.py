@@ -19,7 +19,7 @@ CLI_FAST = os.path.join(HERE, "bin", "cuckoo_fast")
 CLI_FAIL = os.path.join(HERE, "bin", "cuckoo_fail")
 
 CF_OK, CF_E_DEVICE, CF_E_ARGS, CF_E_NOMEM, CF_E_PRIMES, CF_E_UNSUPPORTED = 0, 1, 2, 3, 4, 5
-CF_ALGO_FORWARD, CF_ALGO_CUBIC = 1, 2
+CF_ALGO_FORWARD, CF_ALGO_CUBIC, CF_ALGO_SUFFIX = 1, 2, 3
 
 _lib = None
 u64 = ctypes.c_uint64
@@ -75,6 +75,8 @@ def lib():
         L.cf_last_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
         L.cf_device_name.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
         L.cf_set_graphs.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.cf_run_separate.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.c_int, ctypes.POINTER(u64), ctypes.c_size_t, ctypes.POINTER(u64)]
         L.cf_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tune)]
         L.cf_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.cf_run_fused.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
@@ -234,6 +236,19 @@ class Engine:
         rc = lib().cf_set_variant(self._h, log2_length, variant)
         if rc != CF_OK:
             raise CfError(rc, "bad variant")
+
+    def run_separate(self, n, m, primes, algo="suffix"):
+        """N_bad(n, m) mod p for separate tables of lengths m by the device suffix table (algo
+        "suffix") or forward recurrence (algo "forward")."""
+        code = {"suffix": CF_ALGO_SUFFIX, "forward": CF_ALGO_FORWARD}[algo]
+        np_ = len(primes)
+        P = (u64 * max(np_, 1))(*primes)
+        res = (u64 * max(np_, 1))()
+        mm = (ctypes.c_int32 * max(len(m), 1))(*m)
+        rc = lib().cf_run_separate(self._h, code, n, mm, len(m), P, np_, res)
+        if rc != CF_OK:
+            raise CfError(rc, lib().cf_last_error(self._h).decode())
+        return list(res)[:np_]
 
     def run_algo(self, algo, n, M, primes, d, model=0):
         """N_bad(n, M) mod p for every prime by the device forward (algo "forward", O(n^4)) or cubic

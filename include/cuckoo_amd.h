@@ -110,6 +110,18 @@ int cf_run_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t
 int cf_run_algo(cf_engine* e, int algo, int d, int model, int n, int M, const uint64_t* primes, size_t np,
                 uint64_t* res);
 
+/* The separate-tables model (include/cuckoo/separate_tables.hpp): d tables of lengths m[0..d),
+ * each item one slot per table, Q = prod m_t. res[b] = N_bad(n, m) mod primes[b], one workgroup
+ * per prime. CF_ALGO_SUFFIX: Algorithm 5, the suffix table over the states v >= 1, |v| < p <= n,
+ * with the inner sums by Horner's rule in sigma = x_1 + .. + x_d (separate_tables.hpp:175-291,
+ * build_separate_suffix + SeparateSuffix::nbad), up to C(n-1, d) <= 4e6 states; CF_ALGO_FORWARD:
+ * Algorithm 4, the recurrence over the boundary states (separate_tables.hpp:27-108,
+ * separate_forward), up to 2^20 of them. Selected by `cuckoo_fail separate` (src/cuckoo_fail.cpp:
+ * 83-99). 1 <= d <= 8, m_t >= 1, n < M = sum m_t; primes odd, M < p < 2^50. */
+#define CF_ALGO_SUFFIX 3
+int cf_run_separate(cf_engine* e, int algo, int n, const int32_t* m, int d, const uint64_t* primes, size_t np,
+                    uint64_t* res);
+
 /* The autotuner the batched forms use (no counterpart in the reference, whose CUDA build has one
  * fixed schedule per length, fast/cuckoo_fast.cu:510-646). Each transform-length class 2^10 ..
  * 2^14 has compiled variants of the per-prime kernel that differ in waves per transform (one or

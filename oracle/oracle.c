@@ -370,6 +370,213 @@ int or_nbad_forward(int d, int rep, u64 p, int n, int M, u64* out) {
     return 0;
 }
 
+/* ---- separate tables (separate_tables.hpp): d tables of lengths m_1..m_d, g(v) = prod v_t ---- */
+
+/* All v >= 1 (coordinatewise) with |v| <= smax, by increasing |v|; coordinates in *coord (d per
+ * state), sums in *sum; idx (size (smax+2)^d) maps a vector to its index or -1. */
+typedef struct {
+    int d, smax, N;
+    int *coord, *sum, *idx;
+} States;
+
+static long st_key(const States* S, const int* v) {
+    long k = 0;
+    for (int t = 0; t < S->d; ++t) k = k * (S->smax + 2) + v[t];
+    return k;
+}
+
+static int states_init(States* S, int d, int smax) {
+    S->d = d, S->smax = smax < 0 ? 0 : smax, S->N = 0;
+    long cells = 1;
+    for (int t = 0; t < d; ++t) cells *= S->smax + 2;
+    S->idx = (int*)malloc(sizeof(int) * (size_t)cells);
+    int cap = 1024;
+    S->coord = (int*)malloc(sizeof(int) * (size_t)cap * d);
+    S->sum = (int*)malloc(sizeof(int) * (size_t)cap);
+    if (!S->idx || !S->coord || !S->sum) return -1;
+    for (long c = 0; c < cells; ++c) S->idx[c] = -1;
+    int v[8];
+    for (int s = d; s <= S->smax; ++s) {
+        /* every composition of s into d positive parts, odometer order */
+        for (int t = 0; t < d; ++t) v[t] = 1;
+        v[d - 1] = s - (d - 1);
+        for (;;) {
+            if (S->N == cap) {
+                cap *= 2;
+                S->coord = (int*)realloc(S->coord, sizeof(int) * (size_t)cap * d);
+                S->sum = (int*)realloc(S->sum, sizeof(int) * (size_t)cap);
+                if (!S->coord || !S->sum) return -1;
+            }
+            memcpy(S->coord + (size_t)S->N * d, v, sizeof(int) * d);
+            S->sum[S->N] = s;
+            S->idx[st_key(S, v)] = S->N;
+            S->N++;
+            /* the next composition in lexicographic order: the rightmost part t < d-1 whose tail
+             * (parts t+1 .. d-1) can give up a unit grows by one, the tail restarts at 1, .., 1, rest */
+            int rest = v[d - 1];
+            int t = d - 2;
+            while (t >= 0 && rest <= d - 1 - t) rest += v[t--];
+            if (t < 0) break;
+            v[t]++;
+            rest--;
+            for (int u = t + 1; u < d - 1; ++u) v[u] = 1, rest--;
+            v[d - 1] = rest;
+        }
+    }
+    return 0;
+}
+static void states_free(States* S) { free(S->coord), free(S->sum), free(S->idx); }
+
+/* The suffix table by its defining sums (Algorithm 5's recurrence, separate_tables.hpp:175-186,
+ * without the generating-function evaluation), then N_bad (:190-213). */
+int or_separate_nbad(int n, const int* m, int d, u64 p, u64* out) {
+    if (n < 1 || d < 1 || d > 8) return -1;
+    int M = 0;
+    for (int t = 0; t < d; ++t) {
+        if (m[t] < 1) return -1;
+        M += m[t];
+    }
+    if (M <= n) return -1;
+    Fact F;
+    if (fact_init(&F, M > n ? M : n, p)) return -1;
+    States X;
+    if (states_init(&X, d, n - 1)) return -1;
+    const int N = X.N, P = n + 1;
+    u64* g = (u64*)malloc(sizeof(u64) * (size_t)(N ? N : 1));
+    u64* V = (u64*)calloc((size_t)P * (N ? N : 1), sizeof(u64));  /* V[p][i] */
+    u64* S = (u64*)calloc((size_t)(N ? N : 1) * P, sizeof(u64));  /* S[i][q] */
+    if (!g || !V || !S) return -1;
+    for (int i = 0; i < N; ++i) {
+        g[i] = 1 % p;
+        for (int t = 0; t < d; ++t) g[i] = mulm(g[i], (u64)X.coord[(size_t)i * d + t] % p, p);
+    }
+    for (int pp = 1; pp <= n; ++pp) {
+        const int q0 = pp - 1; /* row q0 of V is final: S[.][q0] from it */
+        if (q0 >= 1)
+            for (int i = 0; i < N; ++i) {
+                if (X.sum[i] < q0) continue;
+                const int* v = X.coord + (size_t)i * d;
+                u64 acc = 0;
+                for (int j = 0; j < N && X.sum[j] < q0; ++j) {
+                    const int* w = X.coord + (size_t)j * d;
+                    int le = 1;
+                    for (int t = 0; t < d && le; ++t) le = w[t] <= v[t];
+                    if (!le) continue;
+                    u64 c = binom(&F, X.sum[i] - X.sum[j] - 1, X.sum[i] - q0, p);
+                    for (int t = 0; t < d && c; ++t) c = mulm(c, binom(&F, v[t], w[t], p), p);
+                    acc = addm(acc, mulm(c, V[(size_t)q0 * N + j], p), p);
+                }
+                S[(size_t)i * P + q0] = acc;
+            }
+        for (int i = 0; i < N && X.sum[i] < pp; ++i) {
+            const int sv = X.sum[i];
+            u64 val = mulm(binom(&F, pp - 1, sv, p), powm(g[i], (u64)pp, p), p);
+            for (int q = 1; q <= sv; ++q) {
+                const u64 c = mulm(mulm(binom(&F, pp - q - 1, pp - sv - 1, p), binom(&F, pp, q, p), p),
+                                   powm(g[i], (u64)(pp - q), p), p);
+                val = subm(val, mulm(c, S[(size_t)i * P + q], p), p);
+            }
+            V[(size_t)pp * N + i] = val;
+        }
+    }
+    u64 Q = 1 % p, total = 0;
+    for (int t = 0; t < d; ++t) Q = mulm(Q, (u64)m[t] % p, p);
+    for (int q = 1; q <= n; ++q) {
+        u64 inner = 0;
+        for (int j = 0; j < N && X.sum[j] < q; ++j) {
+            const int* w = X.coord + (size_t)j * d;
+            u64 c = binom(&F, M - X.sum[j] - 1, M - q, p);
+            for (int t = 0; t < d && c; ++t) c = mulm(c, binom(&F, m[t], w[t], p), p);
+            inner = addm(inner, mulm(c, V[(size_t)q * N + j], p), p);
+        }
+        total = addm(total, mulm(mulm(binom(&F, n, q, p), powm(Q, (u64)(n - q), p), p), inner, p), p);
+    }
+    *out = total;
+    free(g), free(V), free(S);
+    states_free(&X);
+    fact_free(&F);
+    return 0;
+}
+
+/* Algorithm 4, the forward recurrence over (l, b) (separate_tables.hpp:27-108). */
+int or_separate_forward(int n, const int* m, int d, u64 p, u64* out) {
+    if (n < 1 || d < 1 || d > 8) return -1;
+    int M = 0;
+    for (int t = 0; t < d; ++t) {
+        if (m[t] < 1) return -1;
+        M += m[t];
+    }
+    if (M <= n) return -1;
+    const int z = M - n;
+    Fact F;
+    if (fact_init(&F, M, p)) return -1;
+    States X; /* v = m - b >= 1 with |v| <= M - 1 - l; enumerate all v >= 1 with |v| <= M - 1 once */
+    if (states_init(&X, d, M - 1)) return -1;
+    /* the boundary states in order of l: (0, 0), then for l = z .. M-1 every v <= m with |v| < M - l */
+    int cap = 1024, K = 0;
+    int* sl = (int*)malloc(sizeof(int) * (size_t)cap);
+    int* sv = (int*)malloc(sizeof(int) * (size_t)cap); /* index into X, -1 for (0, 0) */
+    if (!sl || !sv) return -1;
+    sl[K] = 0, sv[K] = -1, K++;
+    for (int l = z; l < M; ++l)
+        for (int i = 0; i < X.N && X.sum[i] < M - l; ++i) {
+            const int* v = X.coord + (size_t)i * d;
+            int ok = 1;
+            for (int t = 0; t < d && ok; ++t) ok = v[t] <= m[t];
+            if (!ok) continue;
+            if (K == cap) {
+                cap *= 2;
+                sl = (int*)realloc(sl, sizeof(int) * (size_t)cap);
+                sv = (int*)realloc(sv, sizeof(int) * (size_t)cap);
+                if (!sl || !sv) return -1;
+            }
+            sl[K] = l, sv[K] = i, K++;
+        }
+    int* b = (int*)malloc(sizeof(int) * (size_t)K * d);
+    int* bs = (int*)malloc(sizeof(int) * (size_t)K);
+    u64* h = (u64*)malloc(sizeof(u64) * (size_t)K);
+    u64* Fw = (u64*)calloc((size_t)K, sizeof(u64));
+    if (!b || !bs || !h || !Fw) return -1;
+    for (int k = 0; k < K; ++k) {
+        bs[k] = 0, h[k] = 1 % p;
+        for (int t = 0; t < d; ++t) {
+            const int vt = sv[k] < 0 ? m[t] : X.coord[(size_t)sv[k] * d + t];
+            b[(size_t)k * d + t] = m[t] - vt;
+            bs[k] += m[t] - vt;
+            h[k] = mulm(h[k], (u64)vt % p, p);
+        }
+    }
+#define NU(l) ((l) > z ? (l) - z : 0)
+    Fw[0] = 1 % p;
+    for (int i = 1; i < K; ++i) {
+        u64 s = 0;
+        for (int j = 0; j < i && sl[j] < sl[i]; ++j) {
+            if (!Fw[j] || bs[j] >= bs[i]) continue;
+            int le = 1;
+            for (int t = 0; t < d && le; ++t) le = b[(size_t)j * d + t] <= b[(size_t)i * d + t];
+            if (!le) continue;
+            u64 tau = phi_ab(&F, bs[i] - sl[j], bs[j] - sl[j], sl[i] - sl[j], p);
+            for (int t = 0; t < d && tau; ++t)
+                tau = mulm(tau, binom(&F, m[t] - b[(size_t)j * d + t], b[(size_t)i * d + t] - b[(size_t)j * d + t], p), p);
+            const int u = sl[i] - (sl[j] > z ? sl[j] : z);
+            tau = mulm(tau, mulm(binom(&F, n - NU(sl[j]), u, p), powm(h[j], (u64)u, p), p), p);
+            s = addm(s, mulm(Fw[j], tau, p), p);
+        }
+        Fw[i] = s;
+    }
+    u64 total = 0;
+    for (int i = 0; i < K; ++i)
+        if (Fw[i])
+            total = addm(total, mulm(mulm(Fw[i], binom(&F, M - sl[i] - 1, M - bs[i], p), p),
+                                     powm(h[i], (u64)(n - NU(sl[i])), p), p), p);
+#undef NU
+    *out = total;
+    free(sl), free(sv), free(b), free(bs), free(h), free(Fw);
+    states_free(&X);
+    fact_free(&F);
+    return 0;
+}
+
 /* ---- queries ---- */
 
 static int max_n(int nq, const int* n) {

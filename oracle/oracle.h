@@ -48,6 +48,12 @@ int or_nbad(int algo, int d, int rep, uint64_t p, int nq, const int* n, const in
 /* N_bad(n, M) mod p by Algorithm 1, the O(n^4) forward recurrence over block boundaries
  * (include/cuckoo/single_table.hpp:25-31, 44-78). */
 int or_nbad_forward(int d, int rep, uint64_t p, int n, int M, uint64_t* out);
+/* Separate tables (include/cuckoo/separate_tables.hpp): d tables of lengths m[0..d), one slot per
+ * table per item. or_separate_nbad builds the suffix table V[p][v] over v >= 1, |v| < p <= n from
+ * its defining sums (:175-186) and evaluates N_bad (:190-213); or_separate_forward is Algorithm 4,
+ * the forward recurrence over (l, b) (:27-108). */
+int or_separate_nbad(int n, const int* m, int d, uint64_t p, uint64_t* out);
+int or_separate_forward(int n, const int* m, int d, uint64_t p, uint64_t* out);
 /* N_bad mod p evaluated with the fast engine's query rewrite and Horner finish
  * (fast/cuckoo_fast.cu:306-323, 877-890) on the ntt table. */
 int or_nbad_horner(int d, int rep, uint64_t p, int nq, const int* n, const int* M, uint64_t* out);

@@ -46,6 +46,8 @@ def lib():
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(u64)]
         L.or_nbad_forward.argtypes = [ctypes.c_int, ctypes.c_int, u64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(u64)]
         L.or_brute_force.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_ulonglong)] * 2
+        for f in (L.or_separate_nbad, L.or_separate_forward):
+            f.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, u64, ctypes.POINTER(u64)]
         for f in (L.or_cyclic_conv_schoolbook, L.or_cyclic_conv_ntt):
             f.argtypes = [u64, ctypes.c_size_t, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64)]
         L.or_cyclic_conv_ntt.restype = ctypes.c_int
@@ -110,6 +112,16 @@ def nbad(d, p, queries, rep=False, algo="ntt"):
     if rc:
         raise RuntimeError("oracle nbad failed")
     return list(out)
+
+
+def separate_nbad(n, m, p, algo="suffix"):
+    """N_bad mod p for n items in separate tables of lengths m (list), by the suffix table's
+    defining sums (algo 'suffix') or the forward recurrence (algo 'forward')."""
+    x = u64()
+    f = lib().or_separate_nbad if algo == "suffix" else lib().or_separate_forward
+    if f(n, _arr(ctypes.c_int, m), len(m), p, ctypes.byref(x)):
+        raise RuntimeError("oracle separate-table count failed")
+    return x.value
 
 
 def brute_force(n, M, d, rep=False):

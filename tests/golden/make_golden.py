@@ -10,6 +10,7 @@ the programs' outputs are kept, as data.
     python tests/golden/make_golden.py --only uneven   # just the uneven-batch fixture
     python tests/golden/make_golden.py --only wide     # just the n > 16384 fixture (~2 min, 8 cores)
     python tests/golden/make_golden.py --only algos    # just the --algo forward|cubic fixture
+    python tests/golden/make_golden.py --only separate # just the separate-tables fixture
 """
 import json
 import os
@@ -203,6 +204,36 @@ def fail_algo_cases():
     return recs
 
 
+def fail_separate_cases():
+    """The reference's 32-bit CLI in separate-tables mode (suffix and forward algorithms)."""
+    args = [
+        ["separate", "-n", "10", "-m", "6,6", "--exact"],
+        ["separate", "-n", "10", "-m", "6,6", "--algo", "forward", "--exact"],
+        ["separate", "-n", "30", "-m", "20,25", "--exact"],
+        ["separate", "-n", "40", "-m", "30,30"],
+        ["separate", "-n", "12", "-m", "5,5,5", "--exact"],
+        ["separate", "-n", "15", "-m", "6,7,8", "--algo", "forward"],
+        ["separate", "-n", "20", "-m", "25", "--exact"],
+        ["separate", "-n", "8", "-m", "3,3,3,3", "--exact"],
+        ["separate", "-n", "60", "-m", "40,40", "--mod", "998244353"],
+        ["separate", "-n", "25", "-m", "9,9,9", "--algo", "suffix", "--exact"],
+        ["separate", "-n", "3", "-m", "1,3", "--exact"],
+        ["separate", "-n", "100", "-m", "70,70", "--exact"],
+        ["separate", "-n", "40", "-m", "20,20,20", "--exact"],
+        ["separate", "-n", "10", "-m", "4,5"],
+        ["separate", "-n", "5", "-m", "3,0"],
+        ["separate", "-n", "5", "-m", "3,4", "--algo", "cubic"],
+    ]
+    with ThreadPoolExecutor(max_workers=os.cpu_count()) as ex:
+        outs = list(ex.map(lambda a: run([FAIL, *a]), args))
+    recs = []
+    for a, (rc, so, se) in zip(args, outs):
+        so = re.sub(r"time: [0-9.]+ s", "time: T s", so)
+        so = re.sub(r"\([0-9.]+ s\)", "(T s)", so)
+        recs.append({"args": a, "rc": rc, "stdout": so, "stderr": se})
+    return recs
+
+
 def uneven_golden(n_max=512):
     """The uneven-batch instance list (seed 0) and, for its instances with n <= n_max, the
     reference's residues over the instance's whole prime set (one cuckoo_fast run per instance,
@@ -242,6 +273,10 @@ def main():
         with open(os.path.join(HERE, "uneven_batch.json"), "w") as f:
             json.dump(uneven_golden(), f, indent=0)
         return
+    if only == "separate":
+        with open(os.path.join(HERE, "fail_separate_cases.json"), "w") as f:
+            json.dump(fail_separate_cases(), f, indent=0)
+        return
     if only == "algos":
         with open(os.path.join(HERE, "fail_algo_cases.json"), "w") as f:
             json.dump(fail_algo_cases(), f, indent=0)
@@ -264,6 +299,8 @@ def main():
         json.dump(fail_cases(), f, indent=0)
     with open(os.path.join(HERE, "fail_algo_cases.json"), "w") as f:
         json.dump(fail_algo_cases(), f, indent=0)
+    with open(os.path.join(HERE, "fail_separate_cases.json"), "w") as f:
+        json.dump(fail_separate_cases(), f, indent=0)
     with open(os.path.join(HERE, "fast_16384.json"), "w") as f:
         json.dump(fast_big(16384, 32768, 0, 8), f, indent=0)
     with open(os.path.join(HERE, "uneven_batch.json"), "w") as f:

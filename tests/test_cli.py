@@ -58,9 +58,13 @@ def test_fail_argument_errors(idx):
     assert r.stdout.startswith("usage:")
 
 
-def test_fail_rejects_what_this_build_does_not_run():
-    r = subprocess.run([cf.CLI_FAIL, "separate", "-n", "5", "-m", "3,3"], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 2 and "not available" in r.stderr
+@pytest.mark.parametrize("idx", [i for i, r in enumerate(load("fail_separate_cases.json")) if r["rc"] != 0])
+def test_separate_argument_errors(idx):
+    rec = load("fail_separate_cases.json")[idx]
+    r = subprocess.run([cf.CLI_FAIL, *rec["args"]], capture_output=True, text=True, timeout=60)
+    assert r.returncode == rec["rc"] == 2
+    assert r.stderr.splitlines()[0] == rec["stderr"].splitlines()[0]
+    assert r.stdout.startswith("usage:")
 
 
 # ------------------------------------------------------------------ GPU

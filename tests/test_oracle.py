@@ -1,5 +1,6 @@
 """Pins the CPU oracle before anything is checked against it: golden vectors from the reference's
 own compiled programs, brute force, closed forms and cross-algorithm identities."""
+import math
 import random
 
 import pytest
@@ -241,3 +242,60 @@ def test_forward_oracle_against_brute_force_and_ntt():
         d, rep = rng.randint(1, 5), bool(rng.randint(0, 1))
         p = rng.choice(P_CHECK)
         assert oracle.nbad(d, p, [(n, M)], rep=rep, algo="forward") == oracle.nbad(d, p, [(n, M)], rep=rep)
+
+
+def _separate_brute():
+    import re
+    pat = re.compile(r"\[PASS\] separate\s+n=(\d+) m=\(([\d,]+)\) brute N_bad=(\d+) / (\d+)")
+    return [(int(x.group(1)), [int(v) for v in x.group(2).split(",")], int(x.group(3)), int(x.group(4)))
+            for x in pat.finditer(load("check_ref.txt"))]
+
+
+def test_separate_oracle_against_brute_force():
+    cases = _separate_brute()
+    assert len(cases) >= 20
+    for n, m, nbad, total in cases:
+        q = 1
+        for x in m:
+            q *= x
+        assert total == q ** n
+        for p in P_CHECK + (oracle.prime_range(0, 1)[0],):
+            assert oracle.separate_nbad(n, m, p) == nbad % p, (n, m)
+            assert oracle.separate_nbad(n, m, p, "forward") == nbad % p, (n, m)
+
+
+def test_separate_oracle_against_reference_runs():
+    """The reference CLI's separate-mode N_bad (--exact) reduced mod 50-bit primes, and its --mod
+    residues, equal both oracle algorithms."""
+    primes = oracle.prime_range(0, 2)
+    seen = 0
+    for rec in load("fail_separate_cases.json"):
+        a = rec["args"]
+        if rec["rc"] != 0:
+            continue
+        n = int(a[a.index("-n") + 1])
+        m = [int(x) for x in a[a.index("-m") + 1].split(",")]
+        if "--mod" in a:
+            P = int(a[a.index("--mod") + 1])
+            want = int(rec["stdout"].split(f"N_bad mod {P} = ")[1].split()[0])
+            assert oracle.separate_nbad(n, m, P) == want
+            seen += 1
+            continue
+        nbad = fail_field(rec["stdout"], "N_bad")
+        if nbad is None or n > 40:
+            continue
+        for p in primes:
+            assert oracle.separate_nbad(n, m, p) == int(nbad) % p, a
+            if n <= 15:
+                assert oracle.separate_nbad(n, m, p, "forward") == int(nbad) % p, a
+        seen += 1
+    assert seen >= 8
+
+
+def test_separate_with_one_table_is_the_birthday_count():
+    # d = 1: an item fails only on a collision, N_bad = m^n - m!/(m-n)!
+    p = P_CHECK[0]
+    for n, m in [(3, 5), (7, 20), (12, 13)]:
+        want = (m ** n - math.factorial(m) // math.factorial(m - n)) % p
+        assert oracle.separate_nbad(n, [m], p) == want
+        assert oracle.separate_nbad(n, [m], p, "forward") == want
