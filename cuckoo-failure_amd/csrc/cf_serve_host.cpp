@@ -151,23 +151,42 @@ struct Crt {
     }
     // log2 of the x with x = r_i mod p_i (i < k); -inf for x = 0.
     double log2x(const uint64_t* r, size_t k) const {
-        std::vector<uint64_t> c(k);
-        for (size_t i = 0; i < k; ++i) {
+        double out;
+        log2x_many(&r, &k, 1, &out);
+        return out;
+    }
+    // The same for cnt residue vectors at once, digit by digit in lockstep: each table row is
+    // read once for all of them (the row stays in L1 while the cnt dot products use it), so a
+    // group costs one pass over the table instead of cnt.
+    void log2x_many(const uint64_t* const* r, const size_t* k, size_t cnt, double* out) const {
+        size_t kmax = 0;
+        std::vector<std::vector<uint64_t>> c(cnt);
+        for (size_t q = 0; q < cnt; ++q) c[q].assign(k[q], 0), kmax = std::max(kmax, k[q]);
+        for (size_t i = 0; i < kmax; ++i) {
             const uint64_t* row = tab.data() + i * (i - (i ? 1 : 0)) / 2;
-            unsigned __int128 s = 0;
-            for (size_t j = 0; j < i; ++j) s += (unsigned __int128)c[j] * row[j];
-            const uint64_t pre = (uint64_t)(s % p[i]);
-            const uint64_t t = (r[i] % p[i] + p[i] - pre) % p[i];
-            c[i] = mulmod_u64(t, invP[i], p[i]);
+            for (size_t q = 0; q < cnt; ++q) {
+                if (i >= k[q]) continue;
+                const uint64_t* cq = c[q].data();
+                unsigned __int128 s = 0;
+                for (size_t j = 0; j < i; ++j) s += (unsigned __int128)cq[j] * row[j];
+                const uint64_t pre = (uint64_t)(s % p[i]);
+                const uint64_t t = (r[q][i] % p[i] + p[i] - pre) % p[i];
+                c[q][i] = mulmod_u64(t, invP[i], p[i]);
+            }
         }
-        size_t top = k;
-        while (top > 0 && c[top - 1] == 0) --top;
-        if (!top) return -INFINITY;
-        const size_t i = top - 1;
-        long double lead = (long double)c[i];
-        if (i >= 1) lead += (long double)c[i - 1] / (long double)p[i - 1];
-        if (i >= 2) lead += (long double)c[i - 2] / ((long double)p[i - 1] * (long double)p[i - 2]);
-        return (double)(lp[i] + std::log2(lead));
+        for (size_t q = 0; q < cnt; ++q) {
+            size_t top = k[q];
+            while (top > 0 && c[q][top - 1] == 0) --top;
+            if (!top) {
+                out[q] = -INFINITY;
+                continue;
+            }
+            const size_t i = top - 1;
+            long double lead = (long double)c[q][i];
+            if (i >= 1) lead += (long double)c[q][i - 1] / (long double)p[i - 1];
+            if (i >= 2) lead += (long double)c[q][i - 2] / ((long double)p[i - 1] * (long double)p[i - 2]);
+            out[q] = (double)(lp[i] + std::log2(lead));
+        }
     }
 };
 
@@ -441,20 +460,35 @@ int cf_server_replay(cf_server* s, size_t count, const int32_t* n, const int32_t
     using clock = std::chrono::steady_clock;
     const auto t0 = clock::now();
     auto now_s = [&] { return std::chrono::duration<double>(clock::now() - t0).count(); };
+    // each worker takes up to kCrtGroup finished requests at a time and combines them together
+    constexpr size_t kCrtGroup = 8;
     auto worker = [&] {
+        std::vector<size_t> ids;
+        std::vector<const uint64_t*> rs;
+        std::vector<size_t> ks;
+        std::vector<double> lx;
         for (;;) {
-            size_t i;
+            ids.clear();
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv.wait(lk, [&] { return closing || !crt_q.empty(); });
                 if (crt_q.empty()) return;
-                i = crt_q.front();
-                crt_q.pop_front();
+                while (!crt_q.empty() && ids.size() < kCrtGroup) {
+                    ids.push_back(crt_q.front());
+                    crt_q.pop_front();
+                }
             }
-            const double lx = s->crt->log2x(resid.data() + i * np, (size_t)nd[i]);
-            const double lq = t->model ? log2_q_replacement(M[i], t->d) : log2_q(M[i], t->d);
-            l2[i] = lx - n[i] * lq;
-            done_at[i] = now_s();
+            rs.clear(), ks.clear();
+            for (size_t i : ids) rs.push_back(resid.data() + i * np), ks.push_back((size_t)nd[i]);
+            lx.resize(ids.size());
+            s->crt->log2x_many(rs.data(), ks.data(), ids.size(), lx.data());
+            const double tn = now_s();
+            for (size_t g = 0; g < ids.size(); ++g) {
+                const size_t i = ids[g];
+                const double lq = t->model ? log2_q_replacement(M[i], t->d) : log2_q(M[i], t->d);
+                l2[i] = lx[g] - n[i] * lq;
+                done_at[i] = tn;
+            }
         }
     };
     std::vector<std::thread> pool;
