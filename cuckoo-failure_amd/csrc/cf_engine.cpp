@@ -528,7 +528,10 @@ int cf_device_name(const cf_engine* e, char* buf, size_t len) {
     if (!e || !buf || !len) return CF_E_ARGS;
     hipDeviceProp_t pr;
     if (hipGetDeviceProperties(&pr, e->device) != hipSuccess) return CF_E_DEVICE;
-    snprintf(buf, len, "%s %s", pr.name, pr.gcnArchName);
+    char name[256] = {0};
+    if (hipDeviceGetName(name, sizeof name, e->device) != hipSuccess || !name[0])
+        snprintf(name, sizeof name, "%s", pr.name[0] ? pr.name : "AMD GPU");
+    snprintf(buf, len, "%s %s (%d CUs)", name, pr.gcnArchName, pr.multiProcessorCount);
     return CF_OK;
 }
 
