@@ -323,17 +323,25 @@ struct Engine {
         for (int k = 0; k < 16; ++k) sh.buf[lpos(k * T + t)] = th.x[k];
     }
     // ---- forward round r (1 <= r < NTW) ----
+    // Every round reads its 16 entries (and its 15 twiddles) in a loop of their own before any
+    // arithmetic uses them, so the loads are issued several at a time instead of one load, one
+    // wait, one reduction after another.
     template <class SH>
     static CF_HD void fwdr(int r, Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m) {
         const int lg = P::lgS(r), S = 1 << lg;
         const int b = t >> lg, n2 = t & (S - 1), base = (b << (lg + 4)) + n2;
         double(&x)[16] = th.x;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) x[k] = reduce(sh.buf[lpos(base + k * S)], m);
+        for (int k = 0; k < 16; ++k) x[k] = sh.buf[lpos(base + k * S)];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = reduce(x[k], m);
         dft_groups<16>(x, d.W, m);
         const double* tw = sh.tw + P::tw_off(r);
+        double wt[16];
 #pragma unroll
-        for (int k = 1; k < 16; ++k) x[k] = mulred(x[k], tw[k * S + n2], m);
+        for (int k = 1; k < 16; ++k) wt[k] = tw[k * S + n2];
+#pragma unroll
+        for (int k = 1; k < 16; ++k) x[k] = mulred(x[k], wt[k], m);
 #pragma unroll
         for (int k = 0; k < 16; ++k) sh.buf[lpos(base + k * S)] = x[k];
     }
@@ -346,7 +354,9 @@ struct Engine {
                                   double* hat) {
         double(&x)[16] = th.x;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = reduce(sh.buf[lpos(16 * t + i)], m);
+        for (int i = 0; i < 16; ++i) x[i] = sh.buf[lpos(16 * t + i)];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = reduce(x[i], m);
         dft_groups<P::RF>(x, d.W, m);
         if (make_hat) {
             double* h = hat + 16 * t;
@@ -367,11 +377,13 @@ struct Engine {
         const int b = t >> lg, n2 = t & (S - 1), base = (b << (lg + 4)) + n2;
         double(&x)[16] = th.x;
         const double* tw = sh.tw + P::TWN + P::tw_off(r);
+double wt[16];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const double y = sh.buf[lpos(base + k * S)];
-            x[k] = reduce(k ? mulred(y, tw[k * S + n2], m) : y, m);
-        }
+        for (int k = 0; k < 16; ++k) x[k] = sh.buf[lpos(base + k * S)];
+#pragma unroll
+        for (int k = 1; k < 16; ++k) wt[k] = tw[k * S + n2];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = reduce(k ? mulred(x[k], wt[k], m) : x[k], m);
         dft_groups<16>(x, d.Wi, m);
 #pragma unroll
         for (int k = 0; k < 16; ++k) sh.buf[lpos(base + k * S)] = x[k];
