@@ -34,6 +34,8 @@
 #pragma once
 #include <stdint.h>
 
+#include <type_traits>
+
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define CF_HD __host__ __device__ __forceinline__
@@ -42,6 +44,23 @@
 #endif
 
 namespace cf {
+
+// Compile-time loops: f(std::integral_constant<int, i>) for i = A .. B-1 (static_for) or
+// i = B-1 down to A (static_rfor), so a loop index can be a template argument.
+template <int A, int B, class F>
+CF_HD void static_for(F&& f) {
+    if constexpr (A < B) {
+        f(std::integral_constant<int, A>{});
+        static_for<A + 1, B>(f);
+    }
+}
+template <int A, int B, class F>
+CF_HD void static_rfor(F&& f) {
+    if constexpr (A < B) {
+        f(std::integral_constant<int, B - 1>{});
+        static_rfor<A, B - 1>(f);
+    }
+}
 
 // ------------------------------------------------------------------------------------------
 // Exact modular arithmetic on doubles
@@ -326,9 +345,9 @@ struct Engine {
     // Every round reads its 16 entries (and its 15 twiddles) in a loop of their own before any
     // arithmetic uses them, so the loads are issued several at a time instead of one load, one
     // wait, one reduction after another.
-    template <class SH>
-    static CF_HD void fwdr(int r, Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m) {
-        const int lg = P::lgS(r), S = 1 << lg;
+    template <int r, class SH>
+    static CF_HD void fwdr(Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m) {
+        constexpr int lg = P::lgS(r), S = 1 << lg;
         const int b = t >> lg, n2 = t & (S - 1), base = (b << (lg + 4)) + n2;
         double(&x)[16] = th.x;
 #pragma unroll
@@ -371,9 +390,9 @@ struct Engine {
         for (int i = 0; i < 16; ++i) sh.buf[lpos(16 * t + i)] = x[i];
     }
     // ---- inverse round r (1 <= r < NTW): twiddle w^-(n2 k), then inverse DFT16 ----
-    template <class SH>
-    static CF_HD void invr(int r, Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m) {
-        const int lg = P::lgS(r), S = 1 << lg;
+    template <int r, class SH>
+    static CF_HD void invr(Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m) {
+        constexpr int lg = P::lgS(r), S = 1 << lg;
         const int b = t >> lg, n2 = t & (S - 1), base = (b << (lg + 4)) + n2;
         double(&x)[16] = th.x;
         const double* tw = sh.tw + P::TWN + P::tw_off(r);

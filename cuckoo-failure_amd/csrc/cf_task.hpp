@@ -394,22 +394,26 @@ struct Task {
     static CF_HD void conv_middle(R& run, Sh& sh, int v, int off, const Ctx& c, int X = -1, int k = 0) {
         const bool carry = kDefer && X >= 0;
         // slice k was loaded at the end of the caller's first phase (after its stage input)
-        for (int r = 1; r < P::NTW; ++r)
+        static_for<1, P::NTW>([&](auto RI) {
+            constexpr int r = decltype(RI)::value;
             run.phase([&](Thread& th, int t) {
                 if (r == P::NTW - 1) fetch_hat(th, t, c);
-                E::fwdr(r, th, sh, t, *c.d, c.m);
+                E::template fwdr<r>(th, sh, t, *c.d, c.m);
                 if (carry && r == 1) defer_store(th, sh, t, X, k, c);
             });
+        });
         run.phase([&](Thread& th, int t) {
             E::final_round(false, th, sh, t, *c.d, c.m, c.a.hat);
             if (carry) defer_load(th, t, X, k + 1, c);
         });
-        for (int r = P::NTW - 1; r >= 1; --r)
+        static_rfor<1, P::NTW>([&](auto RI) {
+            constexpr int r = decltype(RI)::value;
             run.phase([&](Thread& th, int t) {
                 if (r == 1) fetch_ifact(th, t, v, off, c);
-                E::invr(r, th, sh, t, *c.d, c.m);
+                E::template invr<r>(th, sh, t, *c.d, c.m);
                 if (carry && r == P::NTW - 1) defer_store(th, sh, t, X, k + 1, c);
             });
+        });
     }
 
     static CF_HD int active_queries(int v, const Ctx& c) {
@@ -484,7 +488,9 @@ struct Task {
             kernel_input(th, t, c);
             E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
         });
-        for (int r = 1; r < P::NTW; ++r) run.phase([&](Thread& th, int t) { E::fwdr(r, th, sh, t, *c.d, c.m); });
+        static_for<1, P::NTW>([&](auto RI) {
+            run.phase([&](Thread& th, int t) { E::template fwdr<decltype(RI)::value>(th, sh, t, *c.d, c.m); });
+        });
         run.phase([&](Thread& th, int t) { E::final_round(true, th, sh, t, *c.d, c.m, c.a.hat); });
 
         // ---- the v-loop ----

@@ -121,18 +121,22 @@ struct Wide {
             inner_load(th, w, g, t);
             E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
         });
-        for (int r = 1; r < P::NTW; ++r)
+        static_for<1, P::NTW>([&](auto RI) {
+            constexpr int r = decltype(RI)::value;
             run.phase([&](Thread& th, int t) {
                 if (!make_hat && r == P::NTW - 1) TK::fetch_hat(th, t, c);
-                E::fwdr(r, th, sh, t, *c.d, c.m);
+                E::template fwdr<r>(th, sh, t, *c.d, c.m);
             });
+        });
         run.phase([&](Thread& th, int t) { E::final_round(make_hat, th, sh, t, *c.d, c.m, c.a.hat); });
         if (make_hat) return;
-        for (int r = P::NTW - 1; r >= 1; --r)
+        static_rfor<1, P::NTW>([&](auto RI) {
+            constexpr int r = decltype(RI)::value;
             run.phase([&](Thread& th, int t) {
                 if (r == 1) TK::fetch_ifact(th, t, v, off, c);
-                E::invr(r, th, sh, t, *c.d, c.m);
+                E::template invr<r>(th, sh, t, *c.d, c.m);
             });
+        });
         run.phase([&](Thread& th, int t) {
             E::inv0(th, sh, t, *c.d, c.m, c.a.tw0);
             inner_store(th, w, g, t);
