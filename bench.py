@@ -195,6 +195,7 @@ def bench_wide(a):
     count = a.primes if a.primes != NPRIMES else 256
     primes = cf.prime_range(0, count)
     eng = cf.Engine()
+    eng.run([(n, M)], primes[:16], d)  # warm-up: one launch, the device arena allocated
     t0 = time.perf_counter()
     got = eng.run([(n, M)], primes, d)
     wall = time.perf_counter() - t0
