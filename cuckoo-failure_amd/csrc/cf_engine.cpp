@@ -447,6 +447,13 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
             unsigned aborted = 0;
             HIP_TRY(e, hipMemcpy(&aborted, u.sync + 2 * nt, sizeof aborted, hipMemcpyDeviceToHost), "download");
             if (aborted) return fail(e, CF_E_DEVICE, "a group barrier of the 2^16 transform timed out");
+            // the XCD each workgroup ran on: a task whose members differ used device-scope barriers
+            std::vector<unsigned> xcc((size_t)kWideMembers * nt);
+            HIP_TRY(e, hipMemcpy(xcc.data(), u.sync + 2 * nt + 2, sizeof(unsigned) * xcc.size(), hipMemcpyDeviceToHost),
+                    "download");
+            for (int k = 0; k < nt; ++k)
+                e->stats.wide_split += !std::all_of(xcc.begin() + kWideMembers * k, xcc.begin() + kWideMembers * (k + 1),
+                                                    [&](unsigned x) { return x == xcc[(size_t)kWideMembers * k]; });
         }
         kernel_ms_total += ms;
         at = end;

@@ -83,6 +83,7 @@ hipError_t launch_queue(int workgroups, int njobs, const QueueJob* jobs, const T
 // all have arrived. The wait is bounded: a waiter that gives up sets the launch's abort flag,
 // which releases every other waiter, and the task ends with an error instead of hanging.
 constexpr long kWideSpinLimit = 1L << 24;
+static_assert(kWideMembers == kWideG, "workgroups per wide task");
 
 struct WideRunner : DevRunner<1, kWideT> {
     unsigned* cnt;    // this task's arrival counter
@@ -91,11 +92,15 @@ struct WideRunner : DevRunner<1, kWideT> {
     unsigned target = 0;
     bool local = false;  // all workgroups of the task share one XCD (and L2)
     __device__ __forceinline__ bool gsync() {
-        if (local) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's stores are in L2
+        // Every wave's loads and stores are complete (its stores in this XCD's L2) before it
+        // arrives: the workgroup barrier waits for LDS operations only, and the fence below runs
+        // in one wave, so without this another wave's stores could miss the write-back and its
+        // loads could still be reading what the next phase's writers overwrite.
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         if (threadIdx.x == 0) {
             target += kWideG;
-            if (!local) __threadfence();  // this workgroup's writes (made complete by the barrier) to the device
+            if (!local) __threadfence();  // this workgroup's writes (all complete in L2 by now) to the device
             __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             int bad = 0;
             long spins = 0;

@@ -46,10 +46,11 @@ hipError_t launch_queue(int workgroups, int njobs, const QueueJob* jobs, const T
 // Tasks with transforms of 2^16, each spread over 16 workgroups that meet at group barriers: one
 // cooperative launch of ntasks tasks (all workgroups resident at once). sync holds
 // wide_sync_words(ntasks) zeros at launch; afterwards sync[2 ntasks] != 0 means a barrier timed
-// out (results invalid). local != 0 lets a task whose workgroups share one XCD use XCD-local
-// barriers (its L2 is then the common cache level).
-constexpr int kWideLog2 = 16, kWideMaxTasks = 16;
-constexpr size_t wide_sync_words(int ntasks) { return 2 * (size_t)ntasks + 2 + 16 * (size_t)ntasks; }
+// out (results invalid), and sync[2 ntasks + 2 + 16 k + g] is the XCD that workgroup g of task k
+// ran on. local != 0 lets a task whose workgroups share one XCD use XCD-local barriers (its L2 is
+// then the common cache level).
+constexpr int kWideLog2 = 16, kWideMaxTasks = 16, kWideMembers = 16;
+constexpr size_t wide_sync_words(int ntasks) { return 2 * (size_t)ntasks + 2 + kWideMembers * (size_t)ntasks; }
 int wide_resident(int device);  // workgroups of the wide kernel the device holds at once
 hipError_t launch_wide(int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem, uint64_t* res,
                        unsigned* sync, int local, hipStream_t stream);

@@ -34,7 +34,8 @@ class CfError(RuntimeError):
 class Stats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double), ("launches", ctypes.c_int),
                 ("tasks", ctypes.c_int), ("batches", ctypes.c_int), ("log2_length", ctypes.c_int),
-                ("device_bytes", ctypes.c_size_t), ("graph", ctypes.c_int)]
+                ("device_bytes", ctypes.c_size_t), ("graph", ctypes.c_int),
+                ("wide_split", ctypes.c_int)]
 
 
 class Instance(ctypes.Structure):

@@ -43,6 +43,8 @@ typedef struct {
     int log2_length; /* log2 of the longest transform of the call */
     size_t device_bytes;
     int graph;       /* 0: launched one by one; 1: captured into a hipGraph and run; 2: graph replayed */
+    int wide_split;  /* tables of more than 16384 rows whose 16 workgroups ran on more than one XCD
+                        (device-scope group barriers instead of XCD-local ones) */
 } cf_stats;
 
 /* Replaces the CUDA back end's device set-up (fast/cuckoo_fast.cu:328-359): selects a HIP device

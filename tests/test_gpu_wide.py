@@ -32,7 +32,7 @@ def test_wide_tables_vs_reference(eng):
         assert st.log2_length == 16
         for i, p, n, M, r in lines:
             b = [x for x, _ in primes].index(i)
-            assert got[b][list(qs).index((n, M))] == r, (d, i, n, M)
+            assert got[b][list(qs).index((n, M))] == r, (d, i, n, M, "tasks split over XCDs", st.wide_split)
 
 
 def test_wide_n65536_reference_prime0(eng):
@@ -49,3 +49,14 @@ def test_wide_mixed_with_short_tables(eng):
     (i, p, n, M, r), = residue_lines(rec["stdout"])
     got = eng.run([(n, M), (200, 400)], [p, 1125899865948161], 3, mask=[[1, 0], [0, 1]])
     assert got[0][0] == r and got[1][1] == 918297158731096
+
+
+def test_wide_device_scope_barriers(eng, monkeypatch):
+    # the device-scope group barrier, the form a task takes when its workgroups span XCDs, forced
+    monkeypatch.setenv("CF_WIDE_DEVICE_BARRIERS", "1")
+    lines = [x for r in load("fast_wide.json") if r["d"] == 3 and [list(q) for q in r["queries"]] == [[16385, 16386]]
+             for x in residue_lines(r["stdout"])]
+    primes = sorted({(i, p) for i, p, _, _, _ in lines})
+    got = eng.run([(16385, 16386)], [p for _, p in primes], 3)
+    want = {i: r for i, _, _, _, r in lines}
+    assert [g[0] for g in got] == [want[i] for i, _ in primes]
