@@ -165,8 +165,8 @@ CF_HD void reduce16(double (&x)[16], const Mod& m) {
 
 // x[k] *= tau^k for k = 1..15 with tau, tau^2, tau^4, tau^8 given (reduced): 11 products build the
 // other powers, 15 apply them.
-CF_HD void twiddle_chain(double (&x)[16], const double* b, int stride, const Mod& m) {
-    const double t1 = b[0], t2 = b[stride], t4 = b[2 * stride], t8 = b[3 * stride];
+CF_HD void twiddle_chain(double (&x)[16], const double (&b)[4], const Mod& m) {
+    const double t1 = b[0], t2 = b[1], t4 = b[2], t8 = b[3];
     const double t3 = mulred(t1, t2, m), t5 = mulred(t4, t1, m), t6 = mulred(t4, t2, m);
     const double t7 = mulred(t4, t3, m);
     const double t9 = mulred(t8, t1, m), t10 = mulred(t8, t2, m), t11 = mulred(t8, t3, m);
@@ -226,11 +226,10 @@ struct Arena {
     double *fact, *ifact, *inv;
     double* qtot;  // per query: Horner accumulator, then Q = g(M)
     double* hat;   // kernel spectrum times 1/L, in the final round's order (L entries)
-    double* tw0;   // round-0 twiddle bases: [8][T], tau^(1,2,4,8) forward then inverse
 };
 
 CF_HD int64_t arena_doubles(int pool, int NS, int F1, int q_cnt, int L) {
-    return (int64_t)pool * 512 + 64 * (int64_t)NS + 3 * (int64_t)F1 + 2 * (int64_t)q_cnt + L + L / 2;
+    return (int64_t)pool * 512 + 64 * (int64_t)NS + 3 * (int64_t)F1 + 2 * (int64_t)q_cnt + L;
 }
 
 CF_HD Arena arena_of(double* base, const TaskDesc& t) {
@@ -244,7 +243,6 @@ CF_HD Arena arena_of(double* base, const TaskDesc& t) {
     a.inv = a.ifact + t.F1;
     a.qtot = a.inv + t.F1;
     a.hat = a.qtot + 2 * (int64_t)t.q_cnt;
-    a.tw0 = a.hat + (int64_t)16 * t.NT;
     return a;
 }
 
@@ -312,6 +310,7 @@ struct Shared {
     using PC = Plan<CAP>;
     double buf[P::LDSW];
     double tw[2 * P::TWN + 2];
+    double wsmall[2][16];  // w^b and w^-b for b < 16 (w the L-th root): the round-0 twiddles with tw
     double lo_pre[64], lo_post[64], hi[PC::HI];
     double red[P::T / 64 * 8 + 8];  // query partial sums: [wave][8]
     double red2[P::T / 64 + 1];     // per-wave sums of the stage-2 input
@@ -333,11 +332,24 @@ struct Engine {
     static constexpr int L = P::L, T = P::T;
 
     // ---- forward round 0 on registers (entries n1 T + t), result stored to LDS ----
+    // tau^(1, 2, 4, 8) for tau = w^t (dir 0) or w^-t (dir 1), t = 16 a + b: w^(16 a) is the
+    // round-1 table's entry (k = 1, n2 = a), w^b comes from wsmall, and three squarings follow.
     template <class SH>
-    static CF_HD void fwd0(Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m, const double* tw0) {
+    static CF_HD void round0_bases(double (&b4)[4], const SH& sh, int t, int dir, const Mod& m) {
+        constexpr int S1 = 1 << P::lgS(1);
+        static_assert(T == 16 * S1, "t = 16 a + b with a < S1");
+        const double* t1 = sh.tw + dir * P::TWN + P::tw_off(1);
+        b4[0] = reduce(mulred(t1[S1 + (t >> 4)], sh.wsmall[dir][t & 15], m), m);
+#pragma unroll
+        for (int i = 1; i < 4; ++i) b4[i] = reduce(mulred(b4[i - 1], b4[i - 1], m), m);
+    }
+    template <class SH>
+    static CF_HD void fwd0(Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m) {
+        double b4[4];
+        round0_bases(b4, sh, t, 0, m);
         reduce16(th.x, m);
         dft_groups<16>(th.x, d.W, m);
-        twiddle_chain(th.x, tw0 + t, T, m);
+        twiddle_chain(th.x, b4, m);
 #pragma unroll
         for (int k = 0; k < 16; ++k) sh.buf[lpos(k * T + t)] = th.x[k];
     }
@@ -409,23 +421,24 @@ double wt[16];
     }
     // ---- inverse round 0: LDS -> registers (entries n1 T + t, natural order) ----
     template <class SH>
-    static CF_HD void inv0(Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m, const double* tw0) {
+    static CF_HD void inv0(Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m) {
         double(&x)[16] = th.x;
+        double b4[4];
 #pragma unroll
         for (int k = 0; k < 16; ++k) x[k] = sh.buf[lpos(k * T + t)];
-        twiddle_chain(x, tw0 + 4 * T + t, T, m);
+        round0_bases(b4, sh, t, 1, m);
+        twiddle_chain(x, b4, m);
         reduce16(x, m);
         dft_groups<16>(x, d.Wi, m);
     }
 
     // ---- prologue pieces ----
-    // round-0 twiddle bases and the LDS tables of rounds >= 1 (entries idx = t, t + T, ...)
+    // w^(+-b) for b < 16 and the LDS tables of rounds >= 1 (entries idx = t, t + T, ...)
     template <class SH>
-    static CF_HD void twiddles(SH& sh, int t, const TaskDesc& d, const Mod& m, double* tw0) {
-        double f = powmod(d.root, (uint64_t)t, m), i = powmod(d.rooti, (uint64_t)t, m);
-        for (int k = 0; k < 4; ++k) {
-            tw0[k * T + t] = f, tw0[(4 + k) * T + t] = i;
-            f = reduce(mulred(f, f, m), m), i = reduce(mulred(i, i, m), m);
+    static CF_HD void twiddles(SH& sh, int t, const TaskDesc& d, const Mod& m) {
+        if (t < 16) {
+            sh.wsmall[0][t] = powmod(d.root, (uint64_t)t, m);
+            sh.wsmall[1][t] = powmod(d.rooti, (uint64_t)t, m);
         }
         for (int r = 1; r < P::NTW; ++r) {
             const int lg = P::lgS(r), S = 1 << lg;

@@ -480,13 +480,13 @@ struct Task {
         run.phase([&](Thread&, int t) { ifact_fix(sh, t, src, c); });
         run.phase([&](Thread&, int t) {
             inv_table(t, c);
-            E::twiddles(sh, t, *c.d, c.m, c.a.tw0);
+            E::twiddles(sh, t, *c.d, c.m);
         });
         // kernel spectrum (times 1/L) into the arena
         run.phase([&](Thread& th, int t) {
             query_init(t, c);
             kernel_input(th, t, c);
-            E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
+            E::fwd0(th, sh, t, *c.d, c.m);
         });
         static_for<1, P::NTW>([&](auto RI) {
             run.phase([&](Thread& th, int t) { E::template fwdr<decltype(RI)::value>(th, sh, t, *c.d, c.m); });
@@ -522,7 +522,7 @@ struct Task {
                             query_partial(th2, t2, v, 0, cnt0, acc, c);
                             return cnt0;
                         }, c.m);
-                        E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
+                        E::fwd0(th, sh, t, *c.d, c.m);
                         if (kDefer && Xd >= 0) defer_load(th, t, Xd, kd, c);
                     });
                 } else {
@@ -532,7 +532,7 @@ struct Task {
                     });
                     queries(run, sh, v, 0, c);
                     run.phase([&](Thread& th, int t) {
-                        E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
+                        E::fwd0(th, sh, t, *c.d, c.m);
                         if (kDefer && Xd >= 0) defer_load(th, t, Xd, kd, c);
                     });
                 }
@@ -541,7 +541,7 @@ struct Task {
                 // power tables of stage 2
                 run.phase([&](Thread& th, int t) {
                     if (nact <= 8 && cnt0) query_finish(sh, t, v, 0, cnt0, c);
-                    E::inv0(th, sh, t, *c.d, c.m, c.a.tw0);
+                    E::inv0(th, sh, t, *c.d, c.m);
                     stage1_post(th, sh, t, v, c);
                     if (s2) powtab(sh, t, v, g, c);
                 });
@@ -560,14 +560,14 @@ struct Task {
                     run.phase([&](Thread& th, int t) {
                         const double bpart = stage2_pre(th, sh, t, v, c);
                         run.wave_partial1(sh.red2, th, t, bpart, c.m);
-                        E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
+                        E::fwd0(th, sh, t, *c.d, c.m);
                         if (kDefer && Xd >= 0) defer_load(th, t, Xd, kd + 2, c);
                     });
                     conv_middle(run, sh, v, 1, c, Xd, kd + 2);
                     run.phase([&](Thread& th, int t) {
                         double B = 0.0;
                         for (int w = 0; w < NW; ++w) B += sh.red2[w];  // NW <= 16 values below p/2 + 4
-                        E::inv0(th, sh, t, *c.d, c.m, c.a.tw0);
+                        E::inv0(th, sh, t, *c.d, c.m);
                         stage2_post(th, sh, t, v, true, reduce(B, c.m), c);
                         if (next) fetch_next(th, sh, t, v, c);
                     });

@@ -57,8 +57,7 @@ struct Wide {
     static constexpr int T = kWideT, G = kWideG, NW = T / 64, N2 = 4096;
     static_assert(P::T == T, "inner plan");
 
-    // The context of workgroup g: the wide element map, its slice of the spectrum and of the
-    // inner round-0 twiddles.
+    // The context of workgroup g: the wide element map and its slice of the spectrum.
     static CF_HD Ctx ctx(const TaskDesc* d, const QueryDesc* qs, const int* slots, double* mem, int g, int model, int dd) {
         Ctx c;
         c.d = d;
@@ -66,7 +65,6 @@ struct Wide {
         c.m.pinv = d->pinv;
         c.a = arena_of(mem, *d);
         c.a.hat += (int64_t)N2 * g;
-        c.a.tw0 += 2048 * (int64_t)g;
         c.slots = slots + d->slot_off;
         c.q = qs;
         c.model = model;
@@ -88,17 +86,27 @@ struct Wide {
             f = reduce(mulred(f, f, m), m), i = reduce(mulred(i, i, m), m);
         }
     }
+    // this thread's w^(n2 2^k), k < 4 (dir 0), or w^-(n2 2^k) (dir 1)
+    static CF_HD void outer_bases(double (&b4)[4], const WideArena& w, int g, int t, int dir) {
+        const double* src = w.twout + 2048 * g + 1024 * dir + t;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) b4[k] = src[k * 256];
+    }
     static CF_HD void outer_fwd(Thread& th, const WideArena& w, int g, int t, const TaskDesc& d, const Mod& m) {
+        double b4[4];
+        outer_bases(b4, w, g, t, 0);
         reduce16(th.x, m);
         dft_groups<16>(th.x, d.W, m);
-        twiddle_chain(th.x, w.twout + 2048 * g + t, 256, m);
+        twiddle_chain(th.x, b4, m);
 #pragma unroll
         for (int k = 0; k < 16; ++k) w.exch[k * N2 + 256 * g + t] = th.x[k];
     }
     static CF_HD void outer_inv(Thread& th, const WideArena& w, int g, int t, const TaskDesc& d, const Mod& m) {
 #pragma unroll
         for (int k = 0; k < 16; ++k) th.x[k] = w.exch[k * N2 + 256 * g + t];
-        twiddle_chain(th.x, w.twout + 2048 * g + 1024 + t, 256, m);
+        double b4[4];
+        outer_bases(b4, w, g, t, 1);
+        twiddle_chain(th.x, b4, m);
         reduce16(th.x, m);
         dft_groups<16>(th.x, d.Wi, m);
     }
@@ -119,7 +127,7 @@ struct Wide {
                             const Ctx& c) {
         run.phase([&](Thread& th, int t) {
             inner_load(th, w, g, t);
-            E::fwd0(th, sh, t, *c.d, c.m, c.a.tw0);
+            E::fwd0(th, sh, t, *c.d, c.m);
         });
         static_for<1, P::NTW>([&](auto RI) {
             constexpr int r = decltype(RI)::value;
@@ -138,7 +146,7 @@ struct Wide {
             });
         });
         run.phase([&](Thread& th, int t) {
-            E::inv0(th, sh, t, *c.d, c.m, c.a.tw0);
+            E::inv0(th, sh, t, *c.d, c.m);
             inner_store(th, w, g, t);
         });
     }
@@ -196,7 +204,7 @@ struct Wide {
         if (run.gsync()) return false;
         run.phase([&](Thread&, int t) {
             TK::inv_table(t, c);
-            E::twiddles(sh, t, d, c.m, c.a.tw0);
+            E::twiddles(sh, t, d, c.m);
             outer_twiddles(w, g, t, d, c.m);
             if (g == 0) TK::query_init(t, c);
         });
