@@ -224,12 +224,13 @@ struct Arena {
     double *Vpool, *Spool;  // pool tiles of 16 x 16 entries, row-major
     double *Vst, *Sst;      // staging: 2 blocks (column block parity) x 16 columns x NS rows, column-major
     double *fact, *ifact, *inv;
+    double* ifact2;  // 1/i!^2: the stage outputs' factorial factors
     double* qtot;  // per query: Horner accumulator, then Q = g(M)
     double* hat;   // kernel spectrum times 1/L, in the final round's order (L entries)
 };
 
 CF_HD int64_t arena_doubles(int pool, int NS, int F1, int q_cnt, int L) {
-    return (int64_t)pool * 512 + 64 * (int64_t)NS + 3 * (int64_t)F1 + 2 * (int64_t)q_cnt + L;
+    return (int64_t)pool * 512 + 64 * (int64_t)NS + 4 * (int64_t)F1 + 2 * (int64_t)q_cnt + L;
 }
 
 CF_HD Arena arena_of(double* base, const TaskDesc& t) {
@@ -241,7 +242,8 @@ CF_HD Arena arena_of(double* base, const TaskDesc& t) {
     a.fact = a.Sst + 32 * (int64_t)t.NS;
     a.ifact = a.fact + t.F1;
     a.inv = a.ifact + t.F1;
-    a.qtot = a.inv + t.F1;
+    a.ifact2 = a.inv + t.F1;
+    a.qtot = a.ifact2 + t.F1;
     a.hat = a.qtot + 2 * (int64_t)t.q_cnt;
     return a;
 }

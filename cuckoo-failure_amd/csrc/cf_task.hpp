@@ -86,7 +86,10 @@ struct Task {
         for (int i = lo; i < hi; ++i) c.a.ifact[i] = reduce(mulred(c.a.ifact[i], f, c.m), c.m);
     }
     static CF_HD void inv_table(int t, const Ctx& c) {
-        for (int i = c.eb + t; i < c.d->F1; i += c.es) c.a.inv[i] = i ? reduce(mulred(c.a.ifact[i], c.a.fact[i - 1], c.m), c.m) : 0.0;
+        for (int i = c.eb + t; i < c.d->F1; i += c.es) {
+            c.a.inv[i] = i ? reduce(mulred(c.a.ifact[i], c.a.fact[i - 1], c.m), c.m) : 0.0;
+            c.a.ifact2[i] = reduce(mulred(c.a.ifact[i], c.a.ifact[i], c.m), c.m);
+        }
     }
     static CF_HD void zero_staging(int t, const Ctx& c) {
         const int64_t n = 32 * (int64_t)c.d->NS;
@@ -144,7 +147,7 @@ struct Task {
 #pragma unroll
         for (int i = 0; i < 16; ++i) th.pa[i] = c.a.hat[16 * t + i];
     }
-    // 1/(e - v - off)! for the stage outputs: off = 0 for stage 1 (rows v <= e < nm), off = 1 for
+    // 1/(e - v - off)!^2 for the stage outputs: off = 0 for stage 1 (rows v <= e < nm), off = 1 for
     // stage 2 (rows v < p <= nm, entry 0 standing for p = L).
     static CF_HD void fetch_ifact(Thread& th, int t, int v, int off, const Ctx& c) {
         const int nm = c.d->nm;
@@ -152,7 +155,7 @@ struct Task {
         for (int k = 0; k < 16; ++k) {
             const int e = k * c.es + c.eb + t, p = (off && e == 0) ? c.L : e;
             const bool ok = off ? (p > v && p <= nm) : (e >= v && e < nm);
-            th.pa[k] = ldo(c.a.ifact, ix8(ok ? p - v - off : 0));
+            th.pa[k] = ldo(c.a.ifact2, ix8(ok ? p - v - off : 0));
         }
     }
     // Stage-1 input a[w] = P2[v][w] / v = V2[v][w] for w < v: columns w < v-1 from th.pa (fetched
@@ -207,7 +210,7 @@ struct Task {
         }
     }
     // Stage-1 output S2[e][v] = c[e] / (e-v)!^2 for v <= e < nm into the staging column v % 16,
-    // with the factorials in th.pa (fetched ahead). Then row v of S2 is fetched into th.pa for the
+    // with 1/(e-v)!^2 in th.pa (fetched ahead). Then row v of S2 is fetched into th.pa for the
     // stage-2 input of the next phase; its entry (v, v) is the value this thread just computed.
     static CF_HD void stage1_post(Thread& th, const Sh& sh, int t, int v, const Ctx& c) {
         double* col = c.a.Sst + st_col(v, c.d->NS);
@@ -216,7 +219,7 @@ struct Task {
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int e = k * c.es + c.eb + t;
-            y[k] = mulred(th.x[k], mulred(th.pa[k], th.pa[k], c.m), c.m);
+            y[k] = mulred(th.x[k], th.pa[k], c.m);
             if (e >= v && e < nm) col[e] = y[k];
         }
         fetch_row(th.pa, c.a.Spool, c.a.Sst, sh, t, v, 1, v, c);
@@ -260,7 +263,7 @@ struct Task {
         return reduce(bsum, c.m);
     }
     // Stage-2 output P2[p][v] = (g^v/v! - B - c'[p]) g^(p-v)/v! / (p-v-1)!^2 for v < p <= nm into
-    // the staging column v % 16 (factorials in th.pa, B the block sum). The entry of row v+1 also
+    // the staging column v % 16 (1/(p-v-1)!^2 in th.pa, B the block sum). The entry of row v+1 also
     // goes to the LDS hand-over for the next step's stage-1 input.
     static CF_HD void stage2_post(const Thread& th, Sh& sh, int t, int v, bool conv, double B, const Ctx& c) {
         double* col = c.a.Vst + st_col(v, c.d->NS);
@@ -271,7 +274,7 @@ struct Task {
             const int e = k * c.es + c.eb + t, p = e ? e : c.L;  // c'[p mod L]: entry 0 stands for p = L
             const bool ok = p > v && p <= nm;
             const int j = ok ? p - v : 1;
-            const double fj = mulred(gpow(sh.lo_post, sh, j, c.m), mulred(th.pa[k], th.pa[k], c.m), c.m);
+            const double fj = mulred(gpow(sh.lo_post, sh, j, c.m), th.pa[k], c.m);
             const double u = s1 - (conv ? reduce(th.x[k], c.m) : 0.0);
             const double y = mulred(u, fj, c.m);
             if (ok) col[p] = y;
