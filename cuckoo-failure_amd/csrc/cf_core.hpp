@@ -341,9 +341,10 @@ struct Engine {
         constexpr int S1 = 1 << P::lgS(1);
         static_assert(T == 16 * S1, "t = 16 a + b with a < S1");
         const double* t1 = sh.tw + dir * P::TWN + P::tw_off(1);
-        b4[0] = reduce(mulred(t1[S1 + (t >> 4)], sh.wsmall[dir][t & 15], m), m);
+        // unreduced products: each below 0.67p, which keeps twiddle_chain's products exact
+        b4[0] = mulred(t1[S1 + (t >> 4)], sh.wsmall[dir][t & 15], m);
 #pragma unroll
-        for (int i = 1; i < 4; ++i) b4[i] = reduce(mulred(b4[i - 1], b4[i - 1], m), m);
+        for (int i = 1; i < 4; ++i) b4[i] = mulred(b4[i - 1], b4[i - 1], m);
     }
     template <class SH>
     static CF_HD void fwd0(Thread& th, SH& sh, int t, const TaskDesc& d, const Mod& m) {
@@ -397,8 +398,13 @@ struct Engine {
             for (int i = 0; i < 16; ++i) h[i] = reduce(mulred(x[i], d.linv, m), m);
             return;
         }
+        // with a radix-2 or radix-4 final round the products (below p: the spectrum entries are
+        // below 2p + 16) enter the inverse DFT unreduced, and two levels keep them below 4p
 #pragma unroll
-        for (int i = 0; i < 16; ++i) x[i] = reduce(mulred(x[i], th.pa[i], m), m);
+        for (int i = 0; i < 16; ++i) {
+            x[i] = mulred(x[i], th.pa[i], m);
+            if (P::RF > 4) x[i] = reduce(x[i], m);
+        }
         dft_groups<P::RF>(x, d.Wi, m);
 #pragma unroll
         for (int i = 0; i < 16; ++i) sh.buf[lpos(16 * t + i)] = x[i];
