@@ -10,10 +10,26 @@
 
 namespace cf {
 
-// Sum over the 64 lanes of a wavefront, reduced after every level (each level at most doubles).
+// A double moved between lanes by a DPP pattern (both halves), all rows and banks enabled.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Sum over the 64 lanes of a wavefront (every lane active), reduced after every level (each level
+// at most doubles). Within a row of 16 lanes the partners come from DPP (quad_perm [1,0,3,2] and
+// [2,3,0,1], then row_half_mirror and row_mirror: after each level every lane of the group holds
+// the group's sum), across rows from two lane shuffles; every lane ends with the total.
 __device__ __forceinline__ double wave_sum(double x, const Mod& m) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) x = reduce(x + __shfl_xor(x, o, 64), m);
+    x = reduce(x + dpp_f64<0xB1>(x), m);   // quad_perm [1,0,3,2]
+    x = reduce(x + dpp_f64<0x4E>(x), m);   // quad_perm [2,3,0,1]
+    x = reduce(x + dpp_f64<0x141>(x), m);  // row_half_mirror: lane i with 7 - i of its half-row
+    x = reduce(x + dpp_f64<0x140>(x), m);  // row_mirror: lane i with 15 - i of its row
+    x = reduce(x + __shfl_xor(x, 16, 64), m);
+    x = reduce(x + __shfl_xor(x, 32, 64), m);
     return x;
 }
 
