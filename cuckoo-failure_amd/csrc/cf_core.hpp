@@ -27,10 +27,10 @@
 //   c     = b (*) k
 //   V2[p][v] = (g^v/v! * 1/p - c[p mod L]) * g^(p-v)/v! / (p-v-1)!^2     v < p <= nm
 // and the query sum of row v (fast/cuckoo_fast.cu:306-310) is sum_w a[w] / (M - w) = out[v] / v!.
-// The table actually stored is P2[p][w] = p V2[p][w]: with B = sum_q b[q] and c' = (q b) (*) k,
-// p c[p] = B + c'[p] (since p/(p-q) = 1 + q/(p-q)), so
-//   P2[p][v] = (g^v/v! - B - c'[p mod L]) * g^(p-v)/v! / (p-v-1)!^2
-// needs no per-entry 1/p, and the row read at step v is scaled back by the one number 1/v.
+// The table actually stored is P2[p][w] = p V2[p][w], so
+//   P2[p][v] = (g^v/v! - p c[p mod L]) * g^(p-v)/v! / (p-v-1)!^2
+// needs no per-entry 1/p (p c[p] is one product by the integer p), and the row read at step v is
+// scaled back by the one number 1/v.
 #pragma once
 #include <stdint.h>
 
@@ -315,7 +315,6 @@ struct Shared {
     double wsmall[2][16];  // w^b and w^-b for b < 16 (w the L-th root): the round-0 twiddles with tw
     double lo_pre[64], lo_post[64], hi[PC::HI];
     double red[P::T / 64 * 8 + 8];  // query partial sums: [wave][8]
-    double red2[P::T / 64 + 1];     // per-wave sums of the stage-2 input
     double scal[8];                 // broadcast scalars
     int segrow[PC::NSEG + 1];       // segment slots of the current row block
     unsigned short dseg[P::NBMAX];  // segments of the tiles whose flush is deferred (one block)

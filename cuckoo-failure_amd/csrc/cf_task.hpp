@@ -248,34 +248,29 @@ struct Task {
     }
     static CF_HD double gpow(const double* lo, const Sh& sh, int e, const Mod& m) { return mulred(sh.hi[e >> 6], lo[e & 63], m); }
 
-    // Stage-2 input: b[q] = S2[v][q] * v! g^(v-q) for 1 <= q <= v (row v of S2 in th.pa); the
-    // transform gets q b[q], and this thread's part of B = sum_q b[q] is returned.
-    static CF_HD double stage2_pre(Thread& th, const Sh& sh, int t, int v, const Ctx& c) {
-        double bsum = 0.0;
+    // Stage-2 input: b[q] = S2[v][q] * v! g^(v-q) for 1 <= q <= v (row v of S2 in th.pa).
+    static CF_HD void stage2_pre(Thread& th, const Sh& sh, int t, int v, const Ctx& c) {
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const int e = k * c.es + c.eb + t, ok = e >= 1 && e <= v, ce = ok ? e : v;
-            const double b = ok ? mulred(th.pa[k], gpow(sh.lo_pre, sh, v - ce, c.m), c.m) : 0.0;
-            bsum += b;
-            if ((k & 3) == 3) bsum = reduce(bsum, c.m);  // four terms below 4.1 p
-            th.x[k] = mulred(b, (double)e, c.m);
+            th.x[k] = ok ? mulred(th.pa[k], gpow(sh.lo_pre, sh, v - ce, c.m), c.m) : 0.0;
         }
-        return reduce(bsum, c.m);
     }
-    // Stage-2 output P2[p][v] = (g^v/v! - B - c'[p]) g^(p-v)/v! / (p-v-1)!^2 for v < p <= nm into
-    // the staging column v % 16 (1/(p-v-1)!^2 in th.pa, B the block sum). The entry of row v+1 also
-    // goes to the LDS hand-over for the next step's stage-1 input.
-    static CF_HD void stage2_post(const Thread& th, Sh& sh, int t, int v, bool conv, double B, const Ctx& c) {
+    // Stage-2 output P2[p][v] = (g^v/v! - p c[p]) g^(p-v)/v! / (p-v-1)!^2 for v < p <= nm into
+    // the staging column v % 16 (c = b (*) k in th.x, 1/(p-v-1)!^2 in th.pa). The entry of row
+    // v+1 also goes to the LDS hand-over for the next step's stage-1 input.
+    static CF_HD void stage2_post(const Thread& th, Sh& sh, int t, int v, bool conv, const Ctx& c) {
         double* col = c.a.Vst + st_col(v, c.d->NS);
         const int nm = c.d->nm;
-        const double s1 = gpow(sh.lo_post, sh, v, c.m) - B;  // g^v / v! - B
+        const double s1 = gpow(sh.lo_post, sh, v, c.m);  // g^v / v!
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const int e = k * c.es + c.eb + t, p = e ? e : c.L;  // c'[p mod L]: entry 0 stands for p = L
+            const int e = k * c.es + c.eb + t, p = e ? e : c.L;  // c[p mod L]: entry 0 stands for p = L
             const bool ok = p > v && p <= nm;
             const int j = ok ? p - v : 1;
             const double fj = mulred(gpow(sh.lo_post, sh, j, c.m), th.pa[k], c.m);
-            const double u = s1 - (conv ? reduce(th.x[k], c.m) : 0.0);
+            // p c[p]: c below 8p + 64 (the inverse transform's last DFT16), p <= 2^16
+            const double u = s1 - (conv ? mulred(th.x[k], (double)p, c.m) : 0.0);
             const double y = mulred(u, fj, c.m);
             if (ok) col[p] = y;
             if (p == v + 1) sh.scal[1] = y;
@@ -561,23 +556,20 @@ struct Task {
                 const bool conv = v > c.v0;  // row v of S is zero unless v > v0
                 if (conv) {
                     run.phase([&](Thread& th, int t) {
-                        const double bpart = stage2_pre(th, sh, t, v, c);
-                        run.wave_partial1(sh.red2, th, t, bpart, c.m);
+                        stage2_pre(th, sh, t, v, c);
                         E::fwd0(th, sh, t, *c.d, c.m);
                         if (kDefer && Xd >= 0) defer_load(th, t, Xd, kd + 2, c);
                     });
                     conv_middle(run, sh, v, 1, c, Xd, kd + 2);
                     run.phase([&](Thread& th, int t) {
-                        double B = 0.0;
-                        for (int w = 0; w < NW; ++w) B += sh.red2[w];  // NW <= 16 values below p/2 + 4
                         E::inv0(th, sh, t, *c.d, c.m);
-                        stage2_post(th, sh, t, v, true, reduce(B, c.m), c);
+                        stage2_post(th, sh, t, v, true, c);
                         if (next) fetch_next(th, sh, t, v, c);
                     });
                 } else {
                     run.phase([&](Thread& th, int t) {
                         fetch_ifact(th, t, v, 1, c);
-                        stage2_post(th, sh, t, v, false, 0.0, c);
+                        stage2_post(th, sh, t, v, false, c);
                         if (next) fetch_next(th, sh, t, v, c);
                     });
                     if (Xd >= 0) defer_phase(run, sh, Xd, kd + 2, kd + 4, c);

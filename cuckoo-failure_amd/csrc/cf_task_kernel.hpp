@@ -69,11 +69,6 @@ struct DevRunner {
 #pragma unroll
             for (int j = 0; j < 8; ++j) red[(t_ >> 6) * 8 + j] = acc[j];
     }
-    // Inside a phase: this wave's sum of x, written by lane 0 to red[wave].
-    __device__ __forceinline__ void wave_partial1(double* red, Thread&, int t_, double x, const Mod& m) {
-        x = wave_sum(x, m);
-        if ((t_ & 63) == 0) red[t_ >> 6] = x;
-    }
     // A phase of its own: the wave sums of f's partials for every virtual thread.
     template <class F>
     __device__ __forceinline__ void wave_reduce8(double* red, F f, const Mod& m) {

@@ -35,17 +35,15 @@ struct WideArena {
     double* exch;   // 16 x 4096 exchange rows
     double* twout;  // per workgroup 2048: w^(n2 2^k) (k < 4) then w^-(n2 2^k), at k 256 + t
     double* qpart;  // [g][kWideQ] per-workgroup query sums
-    double* bpart;  // [g] per-workgroup stage-2 sums
     double* hand;   // [0]: the hand-over entry when it crosses workgroups
 };
-CF_HD int64_t wide_extra_doubles() { return 65536 + 32768 + kWideG * kWideQ + kWideG + 16; }
+CF_HD int64_t wide_extra_doubles() { return 65536 + 32768 + kWideG * kWideQ + 16; }
 CF_HD WideArena wide_arena_of(double* base, const TaskDesc& t) {
     WideArena w;
     w.exch = base + t.arena + arena_doubles(t.pool, t.NS, t.F1, t.q_cnt, 1 << kWideLog2L);
     w.twout = w.exch + 65536;
     w.qpart = w.twout + 32768;
-    w.bpart = w.qpart + kWideG * kWideQ;
-    w.hand = w.bpart + kWideG;
+    w.hand = w.qpart + kWideG * kWideQ;
     return w;
 }
 
@@ -255,34 +253,21 @@ struct Wide {
             if (s2) {
                 if (v > c.v0) {
                     run.phase([&](Thread& th, int t) {
-                        const double bpart = TK::stage2_pre(th, sh, t, v, c);
-                        run.wave_partial1(sh.red2, th, t, bpart, c.m);
+                        TK::stage2_pre(th, sh, t, v, c);
                         outer_fwd(th, w, g, t, d, c.m);
-                    });
-                    run.phase([&](Thread&, int t) {
-                        if (t == 0) {
-                            double s = 0.0;
-                            for (int wv = 0; wv < NW; ++wv) s += sh.red2[wv];
-                            w.bpart[g] = reduce(s, c.m);
-                        }
                     });
                     if (run.gsync()) return false;
                     inner(run, sh, w, g, v, 1, false, c);
                     if (run.gsync()) return false;
                     run.phase([&](Thread& th, int t) {
-                        double B = 0.0;
-                        for (int h = 0; h < G; ++h) {
-                            B += w.bpart[h];
-                            if ((h & 3) == 3) B = reduce(B, c.m);
-                        }
                         outer_inv(th, w, g, t, d, c.m);
-                        TK::stage2_post(th, sh, t, v, true, reduce(B, c.m), c);
+                        TK::stage2_post(th, sh, t, v, true, c);
                         if (next) TK::fetch_next(th, sh, t, v, c);
                     });
                 } else {
                     run.phase([&](Thread& th, int t) {
                         TK::fetch_ifact(th, t, v, 1, c);
-                        TK::stage2_post(th, sh, t, v, false, 0.0, c);
+                        TK::stage2_post(th, sh, t, v, false, c);
                         if (next) TK::fetch_next(th, sh, t, v, c);
                     });
                 }
