@@ -9,10 +9,13 @@
 // the rows 1 .. max n are split into ranges of about equal area, one per wave. A wave walks its
 // range from the top row down, keeping H_j(v) of query j in lane j (H(v-1) = H(v) Q /
 // ((n-v+1)(M-v+1)), so no inverse of Q is needed). A row is read in chunks of 512 entries (8 per
-// lane), the next chunk loaded while the current one is used for every query of the pass;
+// lane), the next chunk loaded while the current one is used for every query of the pass; a
+// chunk that the row fills only partly runs a variant with 1, 2 or 4 entries per lane;
 // 1/(M - w) comes from the prime's inverse table, copied into LDS by the workgroup. Each wave
 // writes one partial sum per query; the finish kernel adds the partials and multiplies by M! n!.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "cf_core.hpp"
 #include "cf_serve.hpp"
@@ -118,20 +121,34 @@ __global__ void __launch_bounds__(kServeThreads, 2)
             if (w0 + 64 * kChunk < v) load(v, w0 + 64 * kChunk);
             else if (v - 1 >= vmin) load(v - 1, 0);
             const int wl = w0 + lane + 64 * (kChunk - 1);
+            // the chunk's products with every query of the pass that uses row v, over its first KK
+            // groups of 64 entries (the groups past the row's end hold zeros and are skipped)
+            auto products = [&](auto KKc) {
+                constexpr int KK = decltype(KKc)::value;
 #pragma unroll
-            for (int j = 0; j < kServeJ; ++j) {
-                if ((on >> j) & 1) {
-                    const int M = __builtin_amdgcn_readlane(qM, j);
-                    const double h = readlane_f64(H, j);
-                    const double* ip = invp + (M - wl);  // entry k at ip[64 (kChunk-1-k)]
-                    double pr[kChunk];
+                for (int j = 0; j < kServeJ; ++j) {
+                    if ((on >> j) & 1) {
+                        const int M = __builtin_amdgcn_readlane(qM, j);
+                        const double h = readlane_f64(H, j);
+                        const double* ip = invp + (M - wl);  // entry k at ip[64 (kChunk-1-k)]
+                        double pr[KK];
 #pragma unroll
-                    for (int k = 0; k < kChunk; ++k) pr[k] = mulred(cur[k], ip[64 * (kChunk - 1 - k)], m);
-                    // tree sum: 8 terms below 0.62 p each stay below 5 p
-                    const double sm = ((pr[0] + pr[1]) + (pr[2] + pr[3])) + ((pr[4] + pr[5]) + (pr[6] + pr[7]));
-                    acc[j] = reduce(acc[j] + mulred(reduce(sm, m), h, m), m);
+                        for (int k = 0; k < KK; ++k) pr[k] = mulred(cur[k], ip[64 * (kChunk - 1 - k)], m);
+                        // tree sum: up to 8 terms below 0.62 p each stay below 5 p
+#pragma unroll
+                        for (int st = 1; st < KK; st <<= 1)
+#pragma unroll
+                            for (int k = 0; k + st < KK; k += 2 * st) pr[k] += pr[k + st];
+                        acc[j] = reduce(acc[j] + mulred(reduce(pr[0], m), h, m), m);
+                    }
                 }
-            }
+            };
+            // groups of 64 entries that reach into row v, rounded up to a power of two (uniform)
+            const int kk = (v - w0 + 63) >> 6;
+            if (kk > 4) products(std::integral_constant<int, 8>{});
+            else if (kk > 2) products(std::integral_constant<int, 4>{});
+            else if (kk > 1) products(std::integral_constant<int, 2>{});
+            else products(std::integral_constant<int, 1>{});
         }
         // H_j(v-1) = H_j(v) Q / ((n-v+1)(M-v+1)) for the queries that used row v (a query whose
         // last row n is below v keeps H_j(n), set at the start)
