@@ -119,11 +119,14 @@ struct Wide {
     }
 
     // The inner part of a convolution (between the two group barriers). The factorials of the
-    // stage output (off: 0 stage 1, 1 stage 2) are fetched into th.pa on the way.
+    // stage output (off: 0 stage 1, 1 stage 2) are fetched into th.pa on the way; with qn > 0 the
+    // first phase also moves this workgroup's query sums of the phase before the barrier (in LDS)
+    // to its slots, which workgroup 0 reads after the next group barrier.
     template <class R>
     static CF_HD void inner(R& run, Sh& sh, const WideArena& w, int g, int v, int off, bool make_hat,
-                            const Ctx& c) {
+                            const Ctx& c, int qn = 0) {
         run.phase([&](Thread& th, int t) {
+            if (qn) query_slots(sh, w, g, t, qn, c.m);
             inner_load(th, w, g, t);
             E::fwd0(th, sh, t, *c.d, c.m);
         });
@@ -233,9 +236,8 @@ struct Wide {
                     query_wave(run, th, sh, t, v, nact, c);
                     outer_fwd(th, w, g, t, d, c.m);
                 });
-                if (nact) run.phase([&](Thread&, int t) { query_slots(sh, w, g, t, nact, c.m); });
                 if (run.gsync()) return false;
-                inner(run, sh, w, g, v, 0, false, c);
+                inner(run, sh, w, g, v, 0, false, c, nact);
                 if (run.gsync()) return false;
                 run.phase([&](Thread& th, int t) {
                     query_total(w, g, t, v, nact, c);
