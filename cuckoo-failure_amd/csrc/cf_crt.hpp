@@ -1,5 +1,6 @@
 // cf_crt.hpp -- exact reconstruction of N_bad from its residues modulo 32-bit primes, for the
-// cuckoo_fail driver (the reference's include/cuckoo/crt.hpp:15-87 does the same job).
+// cuckoo_fail driver (the reference's include/cuckoo/crt.hpp:15-87 does the same job); the decimal
+// N_bad is built on the exact combiner's bignum (cf_bignum.hpp).
 //
 // mixed_radix() gives the unique digits x with N = x0 + x1 p0 + x2 p0 p1 + ...; any exact method
 // yields the same digits. log2_from_digits() must print the same decimal digits as the reference,
@@ -10,6 +11,8 @@
 #include <cstdint>
 #include <string>
 #include <vector>
+
+#include "cf_bignum.hpp"
 
 namespace cfcli {
 
@@ -54,41 +57,11 @@ inline long double log2_from_digits(const std::vector<uint32_t>& p, const std::v
     return static_cast<long double>(ex) + std::log2(mant);
 }
 
-// Natural numbers in base 10^9, little-endian, for the exact decimal output.
-struct Decimal {
-    std::vector<uint32_t> d{0};
-    static constexpr uint32_t B = 1000000000u;
-    void mul_add(uint64_t mul, uint64_t add) {
-        unsigned __int128 carry = add;
-        for (uint32_t& w : d) {
-            const unsigned __int128 v = (unsigned __int128)w * mul + carry;
-            w = (uint32_t)(v % B);
-            carry = v / B;
-        }
-        for (; carry; carry /= B) d.push_back((uint32_t)(carry % B));
-    }
-    void div_exact(uint32_t q) {
-        uint64_t rem = 0;
-        for (size_t i = d.size(); i-- > 0;) {
-            const uint64_t v = rem * B + d[i];
-            d[i] = (uint32_t)(v / q);
-            rem = v % q;
-        }
-        while (d.size() > 1 && d.back() == 0) d.pop_back();
-    }
-    std::string str() const {
-        std::string s = std::to_string(d.back());
-        for (size_t i = d.size() - 1; i-- > 0;) {
-            const std::string w = std::to_string(d[i]);
-            s += std::string(9 - w.size(), '0') + w;
-        }
-        return s;
-    }
-};
-
-inline Decimal from_digits(const std::vector<uint32_t>& p, const std::vector<uint32_t>& x) {
-    Decimal v;
-    for (size_t i = x.size(); i-- > 0;) v.mul_add(p[i], x[i]);
+// N itself from its mixed-radix digits, as a natural number of the exact combiner's bignum:
+// Horner's rule from the top digit, N <- N p_i + x_i.
+inline cfbig::Nat from_mixed_radix(const std::vector<uint32_t>& p, const std::vector<uint32_t>& x) {
+    cfbig::Nat v;
+    for (size_t i = x.size(); i-- > 0;) cfbig::mul_add_small(v, p[i], x[i]);
     return v;
 }
 

@@ -1,17 +1,23 @@
-// cuckoo_fast -- the reference's batch command line (fast/cuckoo_fast.cu:10-16, 653-904) on the
-// MI355X engine:
+// cuckoo_fast -- the batch command line of the reference (fast/cuckoo_fast.cu:10-16, 653-904), run
+// on the MI355X engine:
 //
 //   cuckoo_fast D QUERIES FIRST COUNT [--batch B] [--out FILE]
 //
-// Same arguments, prime list, need(), resume rule, output lines "i p n M residue" (in the same
-// order: primes ascending, queries in file order) and exit codes (2 usage/input, 3 device memory,
-// 4 prime list, 1 device error), so an --out file can be continued by either program and
-// fast/post_gmp.py combines it unchanged. stderr carries a banner and per-batch progress.
+// What a user or a script sees is kept: the arguments, the prime list and need(), the resume rule
+// for --out, the output lines "i p n M residue" (primes ascending, and for each prime the queries
+// it is needed for in file order), the messages of the argument checks and the exit codes
+// (2 usage or input, 3 device memory, 4 prime list, 1 device). So an --out file can be continued
+// by either program, and fast/post_gmp.py combines it unchanged. stderr carries a banner and one
+// progress line per batch.
+//
+// The program is organised as four steps, each a function below: read the command line, read the
+// query file, decide from the ledger of an existing --out file which primes are still to do, then
+// run the work in device-sized batches and append their lines.
 #include <algorithm>
+#include <cerrno>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <set>
 #include <string>
 #include <vector>
 
@@ -20,170 +26,257 @@
 
 namespace {
 
-constexpr int kPrimeBits = 50;
-constexpr int kTwoAdic = 20;
+constexpr int kPrimeBits = 50;  // residues modulo primes c 2^20 + 1 below 2^50
+constexpr int kMaxLog2Rows = 20;
 
-struct Query {
-    int n, M;
-    long need;
+// An early end of the program: the exit status and, unless empty, the line for stderr.
+struct Stop {
+    int status;
+    std::string message;
 };
 
-}  // namespace
-
-int main(int argc, char** argv) {
-    if (argc < 5) {
-        std::fprintf(stderr, "usage: cuckoo_fast D QUERIES FIRST COUNT [--batch B] [--out FILE]\n");
-        return 2;
-    }
-    const int d = std::atoi(argv[1]);
-    const long first = std::atol(argv[3]), count = std::atol(argv[4]);
-    long batch = 0;
+struct CommandLine {
+    int d = 0;
+    long first = 0, count = 0;
+    long batch = 0;  // 0: as many primes as fit in device memory
+    const char* queries_path = nullptr;
     const char* out_path = nullptr;
-    for (int i = 5; i < argc; ++i) {
-        if (!std::strcmp(argv[i], "--batch") && i + 1 < argc) batch = std::atol(argv[++i]);
-        else if (!std::strcmp(argv[i], "--out") && i + 1 < argc) out_path = argv[++i];
-        else {
-            std::fprintf(stderr, "unknown option %s\n", argv[i]);
-            return 2;
-        }
-    }
-    if (first < 0 || count < 0) {
-        std::fprintf(stderr, "FIRST and COUNT must be non-negative\n");
-        return 2;
-    }
+};
 
-    // ---- queries (one "n M" pair per line) ----
-    std::vector<Query> qs;
-    int nmax = 0, Mmax = 0;
-    long maxneed = 0;
-    {
-        FILE* f = std::fopen(argv[2], "r");
-        if (!f) {
-            std::fprintf(stderr, "cannot open %s\n", argv[2]);
-            return 2;
+// Positional arguments are read the way atoi/atol read them (leading number, 0 otherwise).
+long leading_number(const char* s) { return std::strtol(s, nullptr, 10); }
+
+bool read_command_line(int argc, char** argv, CommandLine& cl, Stop& stop) {
+    if (argc < 5) {
+        stop = {2, "usage: cuckoo_fast D QUERIES FIRST COUNT [--batch B] [--out FILE]"};
+        return false;
+    }
+    cl.d = (int)leading_number(argv[1]);
+    cl.queries_path = argv[2];
+    cl.first = leading_number(argv[3]);
+    cl.count = leading_number(argv[4]);
+    int i = 5;
+    while (i < argc) {
+        const bool has_value = i + 1 < argc;
+        if (has_value && std::strcmp(argv[i], "--batch") == 0) {
+            cl.batch = leading_number(argv[i + 1]);
+            i += 2;
+        } else if (has_value && std::strcmp(argv[i], "--out") == 0) {
+            cl.out_path = argv[i + 1];
+            i += 2;
+        } else {
+            stop = {2, std::string("unknown option ") + argv[i]};
+            return false;
         }
+    }
+    if (cl.first < 0 || cl.count < 0) {
+        stop = {2, "FIRST and COUNT must be non-negative"};
+        return false;
+    }
+    return true;
+}
+
+// The query file: whitespace-separated "n M" pairs up to the first token that is not an integer.
+struct QuerySet {
+    std::vector<int32_t> n, M;
+    std::vector<long> need;  // primes 0 .. need-1 carry this query
+    int max_n = 0, max_M = 0;
+    long max_need = 0;
+    size_t size() const { return n.size(); }
+    // how many of the queries prime i is needed for
+    long records_for(long i) const {
+        return (long)std::count_if(need.begin(), need.end(), [i](long k) { return i < k; });
+    }
+};
+
+bool read_queries(const CommandLine& cl, QuerySet& qs, Stop& stop) {
+    FILE* f = std::fopen(cl.queries_path, "r");
+    if (!f) {
+        stop = {2, std::string("cannot open ") + cl.queries_path};
+        return false;
+    }
+    int n = 0, M = 0;
+    bool ok = true;
+    while (ok && std::fscanf(f, "%d %d", &n, &M) == 2) {
+        if (cl.d < 1 || n < 1 || M <= n) {
+            char buf[64];
+            std::snprintf(buf, sizeof buf, "bad query %d %d", n, M);
+            stop = {2, buf};
+            ok = false;
+            break;
+        }
+        qs.n.push_back(n);
+        qs.M.push_back(M);
+        qs.need.push_back(cf_need(n, M, cl.d, kPrimeBits));
+        qs.max_n = std::max(qs.max_n, n);
+        qs.max_M = std::max(qs.max_M, M);
+        qs.max_need = std::max(qs.max_need, qs.need.back());
+    }
+    std::fclose(f);
+    if (!ok) return false;
+    if (qs.size() == 0 || qs.max_n < 8 || qs.max_n >= (1 << kMaxLog2Rows)) {
+        stop = {2, "need queries with 8 <= max n < 2^" + std::to_string(kMaxLog2Rows)};
+        return false;
+    }
+    return true;
+}
+
+// Record counts per prime index of an existing output file. A prime counts as finished when the
+// file holds exactly as many records for its index as there are queries it is needed for
+// (fast/cuckoo_fast.cu:697-715); records are read as long as they parse as five numbers, so a
+// truncated last line ends the ledger, and records of other (n, M) still count.
+class Ledger {
+  public:
+    void load(FILE* f, long limit) {
+        seen_.assign((size_t)std::max(limit, 1L), 0);
+        long i;
+        unsigned long long p, r;
         int n, M;
-        while (std::fscanf(f, "%d %d", &n, &M) == 2) {
-            if (n < 1 || M <= n || d < 1) {
-                std::fprintf(stderr, "bad query %d %d\n", n, M);
-                std::fclose(f);
-                return 2;
-            }
-            qs.push_back({n, M, cf_need(n, M, d, kPrimeBits)});
-            nmax = std::max(nmax, n);
-            Mmax = std::max(Mmax, M);
-            maxneed = std::max(maxneed, qs.back().need);
-        }
-        std::fclose(f);
+        while (std::fscanf(f, "%ld %llu %d %d %llu", &i, &p, &n, &M, &r) == 5)
+            if (i >= 0 && i < limit) ++seen_[(size_t)i];
     }
-    if (qs.empty() || nmax < 8 || nmax + 1 > (1 << kTwoAdic)) {
-        std::fprintf(stderr, "need queries with 8 <= max n < 2^%d\n", kTwoAdic);
-        return 2;
+    bool finished(long i, long wanted) const {
+        return wanted > 0 && i >= 0 && (size_t)i < seen_.size() && seen_[(size_t)i] == wanted;
+    }
+    size_t count_finished(const QuerySet& qs, long from, long to) const {
+        size_t k = 0;
+        for (long i = from; i < to; ++i) k += finished(i, qs.records_for(i));
+        return k;
     }
 
-    // ---- resume: prime i is done iff the file has exactly as many records for i as queries need it ----
-    std::set<long> done;
+  private:
+    std::vector<long> seen_;
+};
+
+struct Plan {
+    std::vector<long> todo;        // prime indices to compute, ascending
+    std::vector<uint64_t> primes;  // primes[i - first] for i in [first, first + count)
     FILE* out = stdout;
-    if (out_path) {
-        if (FILE* f = std::fopen(out_path, "r")) {
-            std::vector<long> cnt((size_t)std::max<long>(first + count, 1), 0);
-            long i;
-            unsigned long long p, r;
-            int n, M;
-            while (std::fscanf(f, "%ld %llu %d %d %llu", &i, &p, &n, &M, &r) == 5)
-                if (i >= 0 && i < (long)cnt.size()) ++cnt[i];
-            std::fclose(f);
-            for (long i2 = first; i2 < first + count; ++i2) {
-                long want = 0;
-                for (const Query& q : qs) want += i2 < q.need;
-                if (want > 0 && cnt[i2] == want) done.insert(i2);
-            }
-        }
-        out = std::fopen(out_path, "a");
-        if (!out) {
-            std::fprintf(stderr, "cannot append to %s\n", out_path);
-            return 2;
-        }
-    }
-    std::vector<long> todo;
-    for (long i = first; i < first + count && i < maxneed; ++i)
-        if (!done.count(i)) todo.push_back(i);
-    if (todo.empty()) {
-        std::fprintf(stderr, "nothing to do (%zu primes already in the output)\n", done.size());
-        return 0;
-    }
-    std::vector<uint64_t> primes((size_t)count);
-    primes.resize((size_t)std::max<long>(cf_prime_range(kPrimeBits, first, count, primes.data()), 0));
-    if (primes.empty() || primes.back() < (uint64_t(1) << (kPrimeBits - 1))) {
-        std::fprintf(stderr, "prime list exhausted above 2^%d\n", kPrimeBits - 1);
-        return 4;
-    }
+};
 
-    // ---- engine and batch size ----
+bool plan_work(const CommandLine& cl, const QuerySet& qs, Plan& plan, Stop& stop) {
+    const long end = cl.first + cl.count;
+    Ledger ledger;
+    bool have_ledger = false;
+    if (cl.out_path) {
+        if (FILE* f = std::fopen(cl.out_path, "r")) {
+            ledger.load(f, end);
+            std::fclose(f);
+            have_ledger = true;
+        }
+        plan.out = std::fopen(cl.out_path, "a");
+        if (!plan.out) {
+            stop = {2, std::string("cannot append to ") + cl.out_path};
+            return false;
+        }
+    }
+    for (long i = cl.first; i < std::min(end, qs.max_need); ++i)
+        if (!(have_ledger && ledger.finished(i, qs.records_for(i)))) plan.todo.push_back(i);
+    if (plan.todo.empty()) {
+        const size_t fin = have_ledger ? ledger.count_finished(qs, cl.first, end) : 0;
+        stop = {0, "nothing to do (" + std::to_string(fin) + " primes already in the output)"};
+        return false;
+    }
+    plan.primes.resize((size_t)cl.count);
+    const long got = cf_prime_range(kPrimeBits, cl.first, cl.count, plan.primes.data());
+    plan.primes.resize((size_t)std::max(got, 0L));
+    if (plan.primes.empty() || plan.primes.back() < (uint64_t(1) << (kPrimeBits - 1))) {
+        stop = {4, "prime list exhausted above 2^" + std::to_string(kPrimeBits - 1)};
+        return false;
+    }
+    return true;
+}
+
+// CF_AUTOTUNE=1: time the kernel variants of the run's transform-length class on its largest n.
+void autotune_if_asked(cf_engine* e, const QuerySet& qs) {
+    const char* tune = std::getenv("CF_AUTOTUNE");
+    if (!tune || tune[0] != '1') return;
+    const int lg = cf_engine_log2_length(qs.max_n, 1 + (uint64_t(1) << 49));
+    cf_tune t;
+    if (lg < 10 || lg > 14) return;
+    const int n_cal = qs.max_n > (1 << (lg - 1)) ? qs.max_n : 0;
+    if (cf_autotune(e, lg, n_cal, &t) == CF_OK)
+        std::fprintf(stderr, "autotune 2^%d on n=%d: variant %d of %d (%.1f ms)\n", lg, t.n, t.chosen, t.variants,
+                     t.ms[t.chosen]);
+}
+
+int run_batches(const CommandLine& cl, const QuerySet& qs, Plan& plan) {
     cf_engine* e = cfcli::open_engine();
     if (!e) return 1;
     // batches of one shape (all but the last) replay one captured graph; CF_GRAPHS=0 turns it off
     const char* graphs = std::getenv("CF_GRAPHS");
     cf_set_graphs(e, !(graphs && graphs[0] == '0'));
+    autotune_if_asked(e, qs);
     const size_t J = qs.size();
-    // CF_AUTOTUNE=1: time the kernel variants of this run's transform length on its largest n
-    const char* tune = std::getenv("CF_AUTOTUNE");
-    if (tune && tune[0] == '1') {
-        const int lg = cf_engine_log2_length(nmax, 1 + (uint64_t(1) << 49));
-        cf_tune t;
-        if (lg >= 10 && lg <= 14 && cf_autotune(e, lg, nmax > (1 << (lg - 1)) ? nmax : 0, &t) == CF_OK)
-            std::fprintf(stderr, "autotune 2^%d on n=%d: variant %d of %d (%.1f ms)\n", lg, t.n, t.chosen, t.variants,
-                         t.ms[t.chosen]);
-    }
-    const size_t per_prime = cf_mem_per_prime(nmax, Mmax, J);
-    if (batch <= 0) batch = cf_max_batch(e, nmax, Mmax, J);
-    if (batch < 1) {
-        std::fprintf(stderr, "not enough device memory: %.2f GB per prime\n", per_prime / 1073741824.0);
+    const double gb = cf_mem_per_prime(qs.max_n, qs.max_M, J) / 1073741824.0;
+    const long fit = cl.batch > 0 ? cl.batch : cf_max_batch(e, qs.max_n, qs.max_M, J);
+    if (fit < 1) {
+        std::fprintf(stderr, "not enough device memory: %.2f GB per prime\n", gb);
         cf_close(e);
         return 3;
     }
-    const size_t B = (size_t)std::min<long>(batch, (long)todo.size());
+    const size_t per_batch = (size_t)std::min<long>(fit, (long)plan.todo.size());
     char dev[128] = "";
     cf_device_name(e, dev, sizeof dev);
     std::fprintf(stderr, "[hip %s] exact FP64 residues, %d-bit primes, d=%d, nmax=%d, L=%zu, %.3f GB per prime, batch %zu, %zu primes to do\n",
-                 dev, kPrimeBits, d, nmax, cf_transform_length((size_t)nmax), per_prime / 1073741824.0, B, todo.size());
+                 dev, kPrimeBits, cl.d, qs.max_n, cf_transform_length((size_t)qs.max_n), gb, per_batch, plan.todo.size());
 
-    std::vector<int32_t> qn(J), qM(J);
-    for (size_t j = 0; j < J; ++j) qn[j] = qs[j].n, qM[j] = qs[j].M;
     const cfcli::Clock clock;
-    for (size_t at = 0; at < todo.size(); at += B) {
-        const size_t nb = std::min(B, todo.size() - at);
-        std::vector<uint64_t> bp(nb), res(nb * J);
-        std::vector<uint8_t> mask(nb * J);
-        int nm = 0;
+    std::vector<uint64_t> bp, res;
+    std::vector<uint8_t> mask;
+    for (size_t at = 0; at < plan.todo.size(); at += per_batch) {
+        const size_t nb = std::min(per_batch, plan.todo.size() - at);
+        const long* idx = plan.todo.data() + at;
+        bp.resize(nb);
+        res.assign(nb * J, 0);
+        mask.resize(nb * J);
+        int rows = 0;
         for (size_t b = 0; b < nb; ++b) {
-            bp[b] = primes[(size_t)(todo[at + b] - first)];
+            bp[b] = plan.primes[(size_t)(idx[b] - cl.first)];
             for (size_t j = 0; j < J; ++j) {
-                mask[b * J + j] = todo[at + b] < qs[j].need;
-                if (mask[b * J + j]) nm = std::max(nm, qs[j].n);
+                mask[b * J + j] = idx[b] < qs.need[j];
+                if (mask[b * J + j]) rows = std::max(rows, (int)qs.n[j]);
             }
         }
-        const int rc = cf_run_fused(e, d, 0, qn.data(), qM.data(), J, bp.data(), nb, mask.data(), res.data());
+        const int rc = cf_run_fused(e, cl.d, 0, qs.n.data(), qs.M.data(), J, bp.data(), nb, mask.data(), res.data());
         if (rc != CF_OK) {
             std::fprintf(stderr, "%s: %s\n", cf_strerror(rc), cf_last_error(e));
             cf_close(e);
             return cfcli::exit_code(rc);
         }
-        std::string lines;
-        char line[160];
+        std::string text;
+        char line[128];
         for (size_t b = 0; b < nb; ++b)
-            for (size_t j = 0; j < J; ++j)
-                if (mask[b * J + j]) {
-                    std::snprintf(line, sizeof line, "%ld %llu %d %d %llu\n", todo[at + b], (unsigned long long)bp[b], qs[j].n,
-                                  qs[j].M, (unsigned long long)res[b * J + j]);
-                    lines += line;
-                }
-        std::fputs(lines.c_str(), out);
-        std::fflush(out);
+            for (size_t j = 0; j < J; ++j) {
+                if (!mask[b * J + j]) continue;
+                std::snprintf(line, sizeof line, "%ld %llu %d %d %llu\n", idx[b], (unsigned long long)bp[b], (int)qs.n[j],
+                              (int)qs.M[j], (unsigned long long)res[b * J + j]);
+                text += line;
+            }
+        std::fputs(text.c_str(), plan.out);
+        std::fflush(plan.out);
         const double t = clock.secs();
-        std::fprintf(stderr, "primes %ld..%ld (n<=%d): build done at %.1f s, queries at %.1f s\n", todo[at], todo[at + nb - 1], nm, t, t);
+        std::fprintf(stderr, "primes %ld..%ld (n<=%d): build done at %.1f s, queries at %.1f s\n", idx[0], idx[nb - 1],
+                     rows, t, t);
     }
-    if (out != stdout) std::fclose(out);
     cf_close(e);
     return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    CommandLine cl;
+    QuerySet qs;
+    Plan plan;
+    Stop stop{0, ""};
+    const bool go = read_command_line(argc, argv, cl, stop) && read_queries(cl, qs, stop) && plan_work(cl, qs, plan, stop);
+    int status = stop.status;
+    if (!go) {
+        if (!stop.message.empty()) std::fprintf(stderr, "%s\n", stop.message.c_str());
+    } else {
+        status = run_batches(cl, qs, plan);
+    }
+    if (plan.out && plan.out != stdout) std::fclose(plan.out);
+    return status;
 }

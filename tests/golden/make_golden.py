@@ -11,6 +11,11 @@ the programs' outputs are kept, as data.
     python tests/golden/make_golden.py --only wide     # just the n > 16384 fixture (~2 min, 8 cores)
     python tests/golden/make_golden.py --only algos    # just the --algo forward|cubic fixture
     python tests/golden/make_golden.py --only separate # just the separate-tables fixture
+    python tests/golden/make_golden.py --only headline   # n = 16384, primes 0..511 (~8 min, 8 cores)
+    python tests/golden/make_golden.py --only widelaunch # n = 32768, primes 0..15 (one 16-task launch)
+    python tests/golden/make_golden.py --only serving    # the serving table set's 41 queries, 1536 primes
+    python tests/golden/make_golden.py --only uneven_big # a sample of the uneven batch with n > 512
+    python tests/golden/make_golden.py --only big        # n = 65536: primes 1..3, and 126 queries on 0..1
 """
 import json
 import os
@@ -266,50 +271,197 @@ def uneven_golden(n_max=512):
     return {"seed": 0, "instances": [list(x) for x in inst], "golden": gold}
 
 
+def _lines_by_prime(stdout):
+    """'i p n M r' lines -> {i: [(p, n, M, r), ...]} in output order."""
+    out = {}
+    for line in stdout.splitlines():
+        t = line.split()
+        if len(t) == 5:
+            out.setdefault(int(t[0]), []).append(tuple(int(x) for x in t[1:]))
+    return out
+
+
+def fast_slice(d, queries, first, count, workers):
+    """One reference process per prime over [first, first + count) (the host build runs one prime
+    per batch anyway), `workers` at a time; returns the concatenated stdout in prime order."""
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        recs = list(ex.map(lambda i: fast_run(d, queries, i, 1), range(first, first + count)))
+    for r in recs:
+        assert r["rc"] == 0, r["stderr"]
+    return "".join(r["stdout"] for r in recs)
+
+
+def headline_slice():
+    """The whole 512-prime slice of the n = 16384, M = 32768, d = 3 workload (about 8 minutes of
+    the IFMA build on 8 cores, 2 GB per process)."""
+    return {"d": 3, "queries": [[16384, 32768]], "first": 0, "count": 512,
+            "stdout": fast_slice(3, [(16384, 32768)], 0, 512, os.cpu_count())}
+
+
+def wide_launch_slice():
+    """16 primes of n = 32768, M = 65536, d = 3: one full 16-task launch of the 2^16 kernel
+    (8.6 GB per process, so at most 5 at once in 62 GB)."""
+    return {"d": 3, "queries": [[32768, 65536]], "first": 0, "count": 16,
+            "stdout": fast_slice(3, [(32768, 65536)], 0, 16, 5)}
+
+
+def queries_126():
+    """126 queries on one n = 65536 table (the query count of the reference's published timing,
+    fast/README.md:80-86): 63 at n = 65536 with M = 65537 + 1040 k, and 63 at n = 1040 (k + 1)
+    with M = 2 n, k = 0..62."""
+    qs = [(65536, 65537 + 1040 * k) for k in range(63)]
+    qs += [(1040 * (k + 1), 2 * 1040 * (k + 1)) for k in range(63)]
+    return qs
+
+
+def big_tables(primes_131072=(1, 2, 3), primes_126=(0, 1)):
+    """n = 65536 tables, one process at a time (32 GB each): more primes of (65536, 131072), and
+    the 126-query file on primes 0 and 1 (about 5 minutes per prime)."""
+    out = {"d": 3, "queries": [[65536, 131072]], "primes": list(primes_131072),
+           "stdout": "".join(fast_run(3, [(65536, 131072)], i, 1)["stdout"] for i in primes_131072)}
+    qs = queries_126()
+    recs = [fast_run(3, qs, i, 1) for i in primes_126]
+    for r in recs:
+        assert r["rc"] == 0, r["stderr"]
+    out126 = {"d": 3, "queries": [list(q) for q in qs], "primes": list(primes_126),
+              "stdout": "".join(r["stdout"] for r in recs)}
+    return out, out126
+
+
+def serving_golden(n_queries=40, count=1536):
+    """The serving workload's table set (d = 3, n_max = 2048, M_max = 8192, primes 0..1535): the
+    reference's residues for the first 40 requests of the seed-1 stream (100k requests) and for
+    (2048, 8192), the largest shape the set answers. One reference run over the prime slice with
+    all 41 queries in one file (lines for i < need(n, M), file order). Per query: need, the first
+    residues, the SHA-256 of the residue list (decimal, one per line, primes ascending), and the
+    CRT-combined exact N_bad's bit length and SHA-256."""
+    import hashlib
+    sys.path.insert(0, os.path.join(REPO, "cuckoo-failure_amd"))
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    import workloads
+
+    n, M, _ = workloads.serving_queries(1, 100000, 32, 2048)
+    qs = [(int(n[i]), int(M[i])) for i in range(n_queries)] + [(2048, 8192)]
+    # one process per 48-prime sub-range (each builds its own table per prime)
+    ranges = [(f, min(48, count - f)) for f in range(0, count, 48)]
+    with ThreadPoolExecutor(max_workers=os.cpu_count()) as ex:
+        recs = list(ex.map(lambda r: fast_run(3, qs, r[0], r[1]), ranges))
+    by_prime = {}
+    for r in recs:
+        assert r["rc"] == 0, r["stderr"]
+        by_prime.update(_lines_by_prime(r["stdout"]))
+    gold = []
+    for j, (qn, qM) in enumerate(qs):
+        k = oracle.need(qn, qM, 3)
+        rows = []
+        for i in range(min(k, count)):
+            hits = [x for x in by_prime.get(i, []) if x[1] == qn and x[2] == qM]
+            assert len(hits) >= 1, (i, qn, qM)
+            rows.append((hits[0][0], hits[0][3]))
+        text = "".join(f"{r}\n" for _, r in rows)
+        N, _ = oracle.crt(rows)
+        raw = N.to_bytes((N.bit_length() + 7) // 8, "little")
+        gold.append({"n": qn, "M": qM, "need": k, "head": [r for _, r in rows[:3]],
+                     "residues_sha256": hashlib.sha256(text.encode()).hexdigest(),
+                     "nbad_bits": N.bit_length(), "nbad_sha256": hashlib.sha256(raw).hexdigest()})
+    return {"d": 3, "nmax": 2048, "Mmax": 8192, "primes": count, "stream_seed": 1, "stream_count": 100000,
+            "queries": gold}
+
+
+def uneven_sample_golden():
+    """The reference's residues for a sample of the uneven batch's instances with n > 512: every
+    5th of them in order of n, and the three largest (one cuckoo_fast run per instance over its
+    whole prime set, 8 at once). Same record as uneven_golden()."""
+    import hashlib
+    sys.path.insert(0, os.path.join(REPO, "cuckoo-failure_amd"))
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    import workloads
+
+    inst = workloads.uneven_batch(0, 256)
+    big = sorted((n, i) for i, (n, d, M) in enumerate(inst) if n > 512)
+    pick = sorted(set([i for _, i in big[::5]] + [i for _, i in big[-3:]]))
+
+    def one(i):
+        n, d, M = inst[i]
+        k = oracle.need(n, M, d)
+        stdout = fast_slice(d, [(n, M)], 0, k, 1)
+        lines = stdout.splitlines()
+        assert len(lines) == k
+        pairs = [(int(t.split()[1]), int(t.split()[4])) for t in lines]
+        N, _ = oracle.crt(pairs)
+        raw = N.to_bytes((N.bit_length() + 7) // 8, "little")
+        return {"index": i, "n": n, "d": d, "M": M, "need": k, "head": lines[:3],
+                "nbad_bits": N.bit_length(), "nbad_sha256": hashlib.sha256(raw).hexdigest()}
+
+    # one instance per worker: each runs its need() primes in one process after the other
+    with ThreadPoolExecutor(max_workers=os.cpu_count()) as ex:
+        gold = list(ex.map(one, pick))
+    return {"seed": 0, "golden": gold}
+
+
+def _write(name, data):
+    """Writes one fixture file after its data is complete (an interrupted run leaves the old file)."""
+    tmp = os.path.join(HERE, name + ".tmp")
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=0)
+    os.replace(tmp, os.path.join(HERE, name))
+
+
 def main():
     os.makedirs(HERE, exist_ok=True)
     only = sys.argv[sys.argv.index("--only") + 1] if "--only" in sys.argv else None
+    if only == "headline":
+        _write("fast_16384.json", headline_slice())
+        return
+    if only == "widelaunch":
+        _write("fast_32768_launch.json", wide_launch_slice())
+        return
+    if only == "serving":
+        _write("serving_2048.json", serving_golden())
+        return
+    if only == "uneven_big":
+        _write("uneven_batch_big.json", uneven_sample_golden())
+        return
+    if only == "big":
+        more, q126 = big_tables()
+        _write("fast_65536_more.json", more)
+        _write("fast_65536_q126.json", q126)
+        return
     if only == "uneven":
-        with open(os.path.join(HERE, "uneven_batch.json"), "w") as f:
-            json.dump(uneven_golden(), f, indent=0)
+        _write("uneven_batch.json", uneven_golden())
         return
     if only == "separate":
-        with open(os.path.join(HERE, "fail_separate_cases.json"), "w") as f:
-            json.dump(fail_separate_cases(), f, indent=0)
+        _write("fail_separate_cases.json", fail_separate_cases())
         return
     if only == "algos":
-        with open(os.path.join(HERE, "fail_algo_cases.json"), "w") as f:
-            json.dump(fail_algo_cases(), f, indent=0)
+        _write("fail_algo_cases.json", fail_algo_cases())
         return
     if only == "wide":
-        with open(os.path.join(HERE, "fast_wide.json"), "w") as f:
-            json.dump(fast_wide(), f, indent=0)
+        _write("fast_wide.json", fast_wide())
         return
     rc, so, _ = run([CHECK])
     assert rc == 0, "reference check program failed"
     with open(os.path.join(HERE, "check_ref.txt"), "w") as f:
         f.write(re.sub(r"\([0-9.]+ s\)", "(T s)", so))
-    with open(os.path.join(HERE, "fast_cases.json"), "w") as f:
-        json.dump(fast_cases(), f, indent=0)
-    with open(os.path.join(HERE, "fast_errors.json"), "w") as f:
-        json.dump(fast_errors(), f, indent=0)
-    with open(os.path.join(HERE, "fast_resume.json"), "w") as f:
-        json.dump(fast_resume(), f, indent=0)
-    with open(os.path.join(HERE, "fail_cases.json"), "w") as f:
-        json.dump(fail_cases(), f, indent=0)
-    with open(os.path.join(HERE, "fail_algo_cases.json"), "w") as f:
-        json.dump(fail_algo_cases(), f, indent=0)
-    with open(os.path.join(HERE, "fail_separate_cases.json"), "w") as f:
-        json.dump(fail_separate_cases(), f, indent=0)
-    with open(os.path.join(HERE, "fast_16384.json"), "w") as f:
-        json.dump(fast_big(16384, 32768, 0, 8), f, indent=0)
-    with open(os.path.join(HERE, "uneven_batch.json"), "w") as f:
-        json.dump(uneven_golden(), f, indent=0)
-    with open(os.path.join(HERE, "fast_wide.json"), "w") as f:
-        json.dump(fast_wide(), f, indent=0)
+    _write("fast_cases.json", fast_cases())
+    _write("fast_errors.json", fast_errors())
+    _write("fast_resume.json", fast_resume())
+    _write("fail_cases.json", fail_cases())
+    _write("fail_algo_cases.json", fail_algo_cases())
+    _write("fail_separate_cases.json", fail_separate_cases())
+    _write("fast_16384.json", headline_slice())
+    _write("uneven_batch.json", uneven_golden())
+    _write("fast_wide.json", fast_wide())
+    _write("fast_32768_launch.json", wide_launch_slice())
+    _write("serving_2048.json", serving_golden())
+    _write("uneven_batch_big.json", uneven_sample_golden())
     if "--big" in sys.argv:
-        with open(os.path.join(HERE, "fast_65536.json"), "w") as f:
-            json.dump(fast_big(65536, 131072, 0, 1), f, indent=0)
+        _write("fast_65536.json", fast_big(65536, 131072, 0, 1))
+        more, q126 = big_tables()
+        _write("fast_65536_more.json", more)
+        _write("fast_65536_q126.json", q126)
 
 
 if __name__ == "__main__":
