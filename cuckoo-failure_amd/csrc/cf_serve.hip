@@ -33,6 +33,7 @@ __device__ __forceinline__ double wave_sum(double x, const Mod& m) {
 constexpr int kWaves = kServeThreads / 64;
 constexpr int kChunk = 8;  // row entries per lane per chunk (512 per wave)
 constexpr int kPad = 64 * kChunk;
+static_assert(kPad == kServeLdsPad, "the host's LDS bound assumes this padding");
 
 __device__ __forceinline__ double readlane_f64(double x, int lane) {
     const uint64_t u = __double_as_longlong(x);

@@ -8,6 +8,10 @@ namespace cf {
 
 constexpr int kServeThreads = 512;  // 8 waves per workgroup, one row range each
 constexpr int kServeJ = 32;         // queries per pass (one accumulator each per lane)
+// The query kernel keeps 1/i for i < F1 = max(nmax, Mmax) + 1, behind 512 zero entries, in LDS
+// (160 KB per CU): table sets with a larger F1 are refused when they are built.
+constexpr int kServeLdsPad = 512;
+constexpr int kServeMaxF1 = 163840 / 8 - kServeLdsPad;
 
 struct ServePrime {
     double p, pinv;

@@ -244,6 +244,10 @@ int cf_tables_build(cf_engine* e, int d, int model, int nmax, int Mmax, const ui
     const auto t_start = std::chrono::steady_clock::now();
     if (d < 1 || (model != 0 && model != 1) || nmax < 1 || Mmax <= nmax || (np && !primes) || !np)
         return fail(e, CF_E_ARGS, "bad table set arguments");
+    if (std::max(nmax, Mmax) + 1 > kServeMaxF1)
+        return fail(e, CF_E_UNSUPPORTED, "max(nmax, Mmax) = " + std::to_string(std::max(nmax, Mmax)) +
+                                             " exceeds the query kernel's LDS table of 1/i (at most " +
+                                             std::to_string(kServeMaxF1 - 1) + ")");
     HIP_TRY(e, hipSetDevice(e->device), "hipSetDevice");
     auto t = std::make_unique<cf_tables>();
     t->e = e;
