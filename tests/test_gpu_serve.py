@@ -124,7 +124,7 @@ def test_server_rejects(tabs):
 
 def test_tables_admission_and_args(eng):
     with pytest.raises(cf.CfError) as ei:
-        cf.Tables(eng, 3, 16384, 32768, cf.prime_range(0, 4000))  # about 4.3 TB: not admitted
+        cf.Tables(eng, 3, 16384, 19000, cf.prime_range(0, 4000))  # about 4.3 TB: not admitted
     assert ei.value.code == cf.CF_E_NOMEM
     with pytest.raises(cf.CfError) as ei:
         cf.Tables(eng, 3, 100, 100, cf.prime_range(0, 2))
