@@ -19,6 +19,7 @@
 #include "../../include/cuckoo_amd.h"
 #include "cf_host.hpp"
 #include "cf_kernels.hpp"
+#include "cf_serve.hpp"
 #include "cf_engine.hpp"
 
 using namespace cf;
@@ -502,6 +503,8 @@ int cf_open(int device, cf_engine** out) {
     HIP_TRY(e.get(), hipGetDeviceCount(&n), "hipGetDeviceCount");
     if (device < 0 || device >= n) return CF_E_DEVICE;
     HIP_TRY(e.get(), hipSetDevice(device), "hipSetDevice");
+    // the device address of cf_query_noalloc's fixed scratch, looked up once per device
+    if (device < cf::kQueryScratchDevices) HIP_TRY(e.get(), cf::attach_query_scratch(device), "hipGetSymbolAddress");
     HIP_TRY(e.get(), hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
     HIP_TRY(e.get(), hipEventCreate(&e->ev0), "hipEventCreate");
     HIP_TRY(e.get(), hipEventCreate(&e->ev1), "hipEventCreate");

@@ -15,6 +15,8 @@
 // writes one partial sum per query; the finish kernel adds the partials and multiplies by M! n!.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <type_traits>
 
 #include "cf_core.hpp"
@@ -186,6 +188,32 @@ __global__ void k_serve_finish(ServeArgs a, uint64_t* __restrict__ res) {
 }
 
 }  // namespace
+
+// The array itself. In host code its name only stands for the symbol: device addresses come from
+// hipGetSymbolAddress, per device, and only those are handed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[kQueryScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kQueryScratchDevices] = {};
+}  // namespace
+
+hipError_t attach_query_scratch(int device) {
+    if (device < 0 || device >= kQueryScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t st = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (st != hipSuccess) return st;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void* query_scratch(int device) {
+    if (device < 0 || device >= kQueryScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
 
 hipError_t launch_serve(const ServeArgs& a, uint64_t* res, hipStream_t stream) {
     if (a.J <= 0 || a.np <= 0) return hipSuccess;

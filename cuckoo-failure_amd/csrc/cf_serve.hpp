@@ -41,4 +41,15 @@ struct ServeArgs {
 // n, largest first.
 hipError_t launch_serve(const ServeArgs& a, uint64_t* res, hipStream_t stream);
 
+// The fixed device scratch of cf_query_noalloc: one statically sized array per device, holding a
+// call's query descriptors, row-range bounds, partial sums and result block. Every part a call
+// reads, it has written first in the same call, so nothing needs clearing between calls.
+constexpr size_t kQueryScratchBytes = size_t(32) << 20;
+constexpr int kQueryScratchDevices = 64;
+// Looks up the array's address on `device`, which must be the current device; once per device
+// (later calls return at once). Called when an engine is opened.
+hipError_t attach_query_scratch(int device);
+// The address found for `device`, or nullptr if it was never attached.
+void* query_scratch(int device);
+
 }  // namespace cf

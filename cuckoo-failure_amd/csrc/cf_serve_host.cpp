@@ -323,7 +323,19 @@ size_t cf_tables_bytes(const cf_tables* t) { return t ? sizeof(double) * (size_t
 
 long cf_tables_need(const cf_tables* t, int n, int M) { return t ? need_of(t, n, M) : -1; }
 
-int cf_query(cf_tables* t, const int32_t* n, const int32_t* M, size_t J, const int64_t* need, uint64_t* res) {
+}  // extern "C"
+
+namespace {
+
+// The per-device lock of cf_query_noalloc: calls on one device share its fixed scratch, so calls
+// from several threads of this process run one after the other.
+std::mutex g_noalloc_mu[kQueryScratchDevices];
+
+// cf_query and cf_query_noalloc: validation, the batch plan, the launch and the result order are
+// the same; `fixed` takes the device scratch from the static array (and caps the range groups so
+// the partial sums fit it) instead of the table set's grown buffer.
+int run_query(cf_tables* t, const int32_t* n, const int32_t* M, size_t J, const int64_t* need, uint64_t* res,
+              bool fixed) {
     if (!t) return CF_E_ARGS;
     cf_engine* e = t->e;
     e->err.clear();
@@ -344,14 +356,38 @@ int cf_query(cf_tables* t, const int32_t* n, const int32_t* M, size_t J, const i
     }
     if (!J) return CF_OK;
     HIP_TRY(e, hipSetDevice(e->device), "hipSetDevice");
-    plan_batch((int)np, 1 << 20, bt);
-    const int R = bt.ranges;
     auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    // scratch layout: queries, bounds, partial sums, results
+    auto layout_bytes = [&](int groups) {
+        const int R = groups * (kServeThreads / 64);
+        return up(sizeof(ServeQuery) * J) + up(sizeof(int) * bounds_count(R, J)) + up(part_bytes(np, R, J)) +
+               sizeof(uint64_t) * np * J;
+    };
+    int max_groups = 1 << 20;
+    if (fixed) {
+        max_groups = groups_for((int)np, t->nmax);
+        while (max_groups > 1 && layout_bytes(max_groups) > kQueryScratchBytes) --max_groups;
+        if (layout_bytes(max_groups) > kQueryScratchBytes)
+            return fail(e, CF_E_ARGS, std::to_string(J) + " queries over " + std::to_string(np) + " primes need " +
+                                          std::to_string(layout_bytes(1)) + " bytes of scratch; the fixed scratch holds " +
+                                          std::to_string(kQueryScratchBytes));
+    }
+    plan_batch((int)np, max_groups, bt);
+    const int R = bt.ranges;
     const size_t o_q = 0, o_b = up(sizeof(ServeQuery) * J), o_p = o_b + up(sizeof(int) * bt.bounds.size());
     const size_t o_r = o_p + up(part_bytes(np, R, J)), total = o_r + sizeof(uint64_t) * np * J;
-    int rc = ensure(e, &t->scratch, &t->scratch_cap, total, "the query scratch");
-    if (rc) return rc;
-    char* sc = (char*)t->scratch;
+    std::unique_lock<std::mutex> lk;
+    char* sc = nullptr;
+    if (fixed) {
+        sc = (char*)query_scratch(e->device);
+        if (!sc) return fail(e, CF_E_DEVICE, "the fixed query scratch was not attached on this device");
+        if (total > kQueryScratchBytes) return fail(e, CF_E_ARGS, "query scratch layout exceeds the fixed scratch");
+        lk = std::unique_lock<std::mutex>(g_noalloc_mu[e->device]);
+    } else {
+        const int rc = ensure(e, &t->scratch, &t->scratch_cap, total, "the query scratch");
+        if (rc) return rc;
+        sc = (char*)t->scratch;
+    }
     HIP_TRY(e, hipMemcpyAsync(sc + o_q, bt.q.data(), sizeof(ServeQuery) * J, hipMemcpyHostToDevice, t->stream), "upload");
     HIP_TRY(e, hipMemcpyAsync(sc + o_b, bt.bounds.data(), sizeof(int) * bt.bounds.size(), hipMemcpyHostToDevice, t->stream),
             "upload");
@@ -369,6 +405,7 @@ int cf_query(cf_tables* t, const int32_t* n, const int32_t* M, size_t J, const i
     HIP_TRY(e, hipEventRecord(e->ev1, t->stream), "hipEventRecord");
     HIP_TRY(e, hipMemcpyAsync(sorted.data(), sc + o_r, sizeof(uint64_t) * np * J, hipMemcpyDeviceToHost, t->stream), "download");
     HIP_TRY(e, hipStreamSynchronize(t->stream), "query execution");
+    if (lk.owns_lock()) lk.unlock();
     for (size_t b = 0; b < np; ++b)
         for (size_t k = 0; k < J; ++k) res[b * J + bt.order[k]] = sorted[b * J + k];
     float ms = 0;
@@ -378,6 +415,18 @@ int cf_query(cf_tables* t, const int32_t* n, const int32_t* M, size_t J, const i
     e->stats.tasks = (int)np;
     e->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return CF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cf_query(cf_tables* t, const int32_t* n, const int32_t* M, size_t J, const int64_t* need, uint64_t* res) {
+    return run_query(t, n, M, J, need, res, false);
+}
+
+int cf_query_noalloc(cf_tables* t, const int32_t* n, const int32_t* M, size_t J, const int64_t* need, uint64_t* res) {
+    return run_query(t, n, M, J, need, res, true);
 }
 
 double cf_crt_log2(const uint64_t* primes, const uint64_t* res, size_t k) {

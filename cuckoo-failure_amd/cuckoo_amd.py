@@ -98,6 +98,7 @@ def lib():
         L.cf_tables_need.restype = ctypes.c_long
         L.cf_query.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t,
                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(u64)]
+        L.cf_query_noalloc.argtypes = L.cf_query.argtypes
         L.cf_crt_log2.argtypes = [ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.c_size_t]
         L.cf_crt_log2.restype = ctypes.c_double
         L.cf_exact_combine.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(u64), ctypes.POINTER(u64),
@@ -332,14 +333,15 @@ class Tables:
     def need(self, n, M):
         return lib().cf_tables_need(self._h, n, M)
 
-    def query(self, queries, need=None):
-        """res[b][j] = N_bad(n_j, M_j) mod p_b for b < need[j] (None: all primes), else 0."""
+    def query(self, queries, need=None, noalloc=False):
+        """res[b][j] = N_bad(n_j, M_j) mod p_b for b < need[j] (None: all primes), else 0. With
+        noalloc, through cf_query_noalloc (no device allocation; fixed per-device scratch)."""
         J, np_ = len(queries), len(self.primes)
         n = (ctypes.c_int32 * max(J, 1))(*[q[0] for q in queries])
         M = (ctypes.c_int32 * max(J, 1))(*[q[1] for q in queries])
         nd = None if need is None else (ctypes.c_int64 * max(J, 1))(*need)
         res = (u64 * max(np_ * J, 1))()
-        rc = lib().cf_query(self._h, n, M, J, nd, res)
+        rc = (lib().cf_query_noalloc if noalloc else lib().cf_query)(self._h, n, M, J, nd, res)
         if rc != CF_OK:
             raise CfError(rc, lib().cf_last_error(self.engine._h).decode())
         return [[res[b * J + j] for j in range(J)] for b in range(np_)]

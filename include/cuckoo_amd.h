@@ -173,6 +173,18 @@ long cf_tables_need(const cf_tables* t, int n, int M);
  * b < need[j] (need NULL: every prime), else 0. n_j <= nmax, n_j < M_j <= Mmax. */
 int cf_query(cf_tables* t, const int32_t* n, const int32_t* M, size_t J, const int64_t* need, uint64_t* res);
 
+/* cf_query without any device allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments, checks and results as cf_query; the query descriptors, row
+ * ranges, partial sums and results of a call live in one fixed 32 MiB array per device, compiled
+ * into the library (held until the process ends), instead of the set's grown buffer. Its device
+ * address is looked up when an engine is opened (devices 0..63). CF_E_ARGS when a call would need
+ * more than the 32 MiB: about 110 KB per query at 1536 primes, so up to ~290 queries per call
+ * there. The array keeps the last call's contents, and every part a call reads it writes first.
+ * Calls on one device share the array: this library runs them one after the other (a per-device
+ * lock held until the call's results are back), so only another process could race them. */
+int cf_query_noalloc(cf_tables* t, const int32_t* n, const int32_t* M, size_t J, const int64_t* need, uint64_t* res);
+
 /* log2 of the x in [0, prod p) with x = res[i] mod primes[i] (host CRT by Garner's mixed radix,
  * as fast/post_gmp.py's CRT merge, in long double from the leading digits); -inf for x = 0. */
 double cf_crt_log2(const uint64_t* primes, const uint64_t* res, size_t k);
