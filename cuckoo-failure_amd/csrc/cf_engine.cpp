@@ -92,8 +92,6 @@ int plan_task(cf_engine* e, uint64_t p, int d, int model, size_t nq, Q query, Ta
     if (tp.lg > kMaxLog2L && tp.lg != kWideLog2)
         return fail(e, CF_E_UNSUPPORTED, "n = " + std::to_string(tp.nm) + " needs a transform longer than 2^" +
                                              std::to_string(kWideLog2) + ", which this build does not run");
-    if (tp.lg == kWideLog2 && tp.queries.size() > 8)
-        return fail(e, CF_E_UNSUPPORTED, "more than 8 queries on one table of more than 16384 rows");
     tp.bytes = task_bytes(plan_for(e, tp.nm), tp.F1, (int)tp.queries.size(), tp.lg);
     // two convolutions of length L per row, plus the tile traffic
     tp.cost = 2.0 * tp.nm * ((double)(1 << tp.lg) * (tp.lg + 1)) + 8.0 * tp.nm * (double)tp.nm;

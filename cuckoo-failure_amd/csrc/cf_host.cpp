@@ -168,7 +168,7 @@ SlotPlan plan_slots(int nm) {
 
 size_t task_bytes(const SlotPlan& sp, int F1, int q_cnt, int log2L) {
     // a 2^16 task also holds its exchange rows, outer twiddles and per-workgroup sums
-    const int64_t extra = log2L == 16 ? 65536 + 32768 + 16 * 8 + 16 : 0;
+    const int64_t extra = log2L == 16 ? wide_extra_doubles(q_cnt) : 0;
     return 8 * (size_t)(arena_doubles(sp.pool, sp.NS, F1, q_cnt, 1 << log2L) + extra);
 }
 

@@ -13,8 +13,9 @@
 //          inverse rounds), storing the row back;
 //   group barrier;
 //   outer inverse: load the thread's column, twiddle w^-(n2 k1), inverse DFT16.
-// Sums over all entries (query terms, the stage-2 block sum) are reduced per workgroup, written
-// to small per-workgroup slots and added up after the next group barrier. The factorial tables
+// Query terms are reduced per workgroup, written to per-workgroup slots (one per query) and added
+// up by workgroup 0 after the next group barrier; up to 8 queries ride in the stage-1 input phase,
+// more take two phases per group of 8 of their own before the outer transform. The factorial tables
 // are built by workgroup 0; staging zeroing and the tile flush are split over all workgroups.
 //
 // Reference correspondence is that of cf_task.hpp (fast/cuckoo_fast.cu:787-898); only the
@@ -28,22 +29,23 @@ constexpr int kWideG = 16;            // workgroups per prime
 constexpr int kWideLog2L = 16;        // L = 2^16
 constexpr int kWideInner = 12;        // inner transform length 2^12 = 4096
 constexpr int kWideT = 256;           // threads per workgroup
-constexpr int kWideQ = 8;             // queries per wide task (at most)
+static_assert(kWideG == 16, "wide_extra_doubles() assumes 16 workgroups per prime");
 
 // Extra arena of a wide task, after the one-workgroup layout of length 2^16 (doubles).
 struct WideArena {
     double* exch;   // 16 x 4096 exchange rows
     double* twout;  // per workgroup 2048: w^(n2 2^k) (k < 4) then w^-(n2 2^k), at k 256 + t
-    double* qpart;  // [g][kWideQ] per-workgroup query sums
+    double* qpart;  // [g][qs] per-workgroup query sums
     double* hand;   // [0]: the hand-over entry when it crosses workgroups
+    int qs;         // query slots per workgroup: max(8, the task's query count)
 };
-CF_HD int64_t wide_extra_doubles() { return 65536 + 32768 + kWideG * kWideQ + 16; }
 CF_HD WideArena wide_arena_of(double* base, const TaskDesc& t) {
     WideArena w;
     w.exch = base + t.arena + arena_doubles(t.pool, t.NS, t.F1, t.q_cnt, 1 << kWideLog2L);
     w.twout = w.exch + 65536;
     w.qpart = w.twout + 32768;
-    w.hand = w.qpart + kWideG * kWideQ;
+    w.qs = wide_qslots(t.q_cnt);
+    w.hand = w.qpart + (int64_t)kWideG * w.qs;
     return w;
 }
 
@@ -120,13 +122,13 @@ struct Wide {
 
     // The inner part of a convolution (between the two group barriers). The factorials of the
     // stage output (off: 0 stage 1, 1 stage 2) are fetched into th.pa on the way; with qn > 0 the
-    // first phase also moves this workgroup's query sums of the phase before the barrier (in LDS)
-    // to its slots, which workgroup 0 reads after the next group barrier.
+    // first phase also moves this workgroup's sums of queries 0 .. qn-1 of the phase before the
+    // barrier (in LDS) to its slots, which workgroup 0 reads after the next group barrier.
     template <class R>
     static CF_HD void inner(R& run, Sh& sh, const WideArena& w, int g, int v, int off, bool make_hat,
                             const Ctx& c, int qn = 0) {
         run.phase([&](Thread& th, int t) {
-            if (qn) query_slots(sh, w, g, t, qn, c.m);
+            if (qn) query_slots(sh, w, g, t, 0, qn, c.m);
             inner_load(th, w, g, t);
             E::fwd0(th, sh, t, *c.d, c.m);
         });
@@ -152,32 +154,43 @@ struct Wide {
         });
     }
 
-    // Query terms of row v (th.x, stage-1 input form) for queries 0 .. cnt-1: this workgroup's
-    // wave sums to LDS (inside the caller's phase), then to the workgroup's slots.
+    // Query terms of row v (th.x, stage-1 input form) for queries j0 .. j0+cnt-1 (cnt <= 8): this
+    // workgroup's wave sums to LDS (inside the caller's phase), then to the workgroup's slots.
     template <class R>
-    static CF_HD void query_wave(R& run, Thread& th, Sh& sh, int t, int v, int cnt, const Ctx& c) {
+    static CF_HD void query_wave(R& run, Thread& th, Sh& sh, int t, int v, int j0, int cnt, const Ctx& c) {
         if (cnt)
             run.wave_partial8(sh.red, th, t, [&](Thread& th2, int t2, double (&acc)[8]) {
-                TK::query_partial(th2, t2, v, 0, cnt, acc, c);
+                TK::query_partial(th2, t2, v, j0, cnt, acc, c);
                 return cnt;
             }, c.m);
     }
-    static CF_HD void query_slots(const Sh& sh, const WideArena& w, int g, int t, int cnt, const Mod& m) {
+    static CF_HD void query_slots(const Sh& sh, const WideArena& w, int g, int t, int j0, int cnt, const Mod& m) {
         if (t < cnt) {
             double s = 0.0;
             for (int wv = 0; wv < NW; ++wv) s += sh.red[wv * 8 + t];  // 4 values below p/2 + 4
-            w.qpart[g * kWideQ + t] = reduce(s, m);
+            w.qpart[(int64_t)g * w.qs + j0 + t] = reduce(s, m);
+        }
+    }
+    // Queries beyond the first 8 of row v: per group of 8, the wave sums in one phase and the
+    // workgroup's slots in the next (th.x keeps the stage-1 input throughout).
+    template <class R>
+    static CF_HD void query_groups(R& run, Sh& sh, const WideArena& w, int g, int v, int nact, const Ctx& c) {
+        for (int j0 = 8; j0 < nact; j0 += 8) {
+            const int cnt = nact - j0 < 8 ? nact - j0 : 8;
+            run.phase([&](Thread& th, int t) { query_wave(run, th, sh, t, v, j0, cnt, c); });
+            run.phase([&](Thread&, int t) { query_slots(sh, w, g, t, j0, cnt, c.m); });
         }
     }
     // Workgroup 0 finishes the queries from all workgroups' slots (after a group barrier).
     static CF_HD void query_total(const WideArena& w, int g, int t, int v, int cnt, const Ctx& c) {
-        if (g == 0 && t < cnt) {
+        if (g != 0) return;
+        for (int j = t; j < cnt; j += T) {
             double s = 0.0;
             for (int h = 0; h < G; ++h) {
-                s += w.qpart[h * kWideQ + t];
+                s += w.qpart[(int64_t)h * w.qs + j];
                 if ((h & 3) == 3) s = reduce(s, c.m);
             }
-            TK::query_update(t, v, 0, reduce(s, c.m), c);
+            TK::query_update(j, v, 0, reduce(s, c.m), c);
         }
     }
 
@@ -229,15 +242,26 @@ struct Wide {
                 run.phase([&](Thread&, int t) { TK::load_segrow(sh, t, segY, c); });
             }
             if (s1) {
-                const int nact = TK::active_queries(v, c);  // <= kWideQ
+                const int nact = TK::active_queries(v, c);
                 const bool now = !pref;
-                run.phase([&](Thread& th, int t) {
-                    TK::stage1_pre(th, sh, t, v, now, c);
-                    query_wave(run, th, sh, t, v, nact, c);
-                    outer_fwd(th, w, g, t, d, c.m);
-                });
+                if (nact <= 8) {
+                    // the stage-1 input, its query terms and the outer transform in one phase
+                    run.phase([&](Thread& th, int t) {
+                        TK::stage1_pre(th, sh, t, v, now, c);
+                        query_wave(run, th, sh, t, v, 0, nact, c);
+                        outer_fwd(th, w, g, t, d, c.m);
+                    });
+                } else {
+                    run.phase([&](Thread& th, int t) {
+                        TK::stage1_pre(th, sh, t, v, now, c);
+                        query_wave(run, th, sh, t, v, 0, 8, c);
+                    });
+                    run.phase([&](Thread&, int t) { query_slots(sh, w, g, t, 0, 8, c.m); });
+                    query_groups(run, sh, w, g, v, nact, c);
+                    run.phase([&](Thread& th, int t) { outer_fwd(th, w, g, t, d, c.m); });
+                }
                 if (run.gsync()) return false;
-                inner(run, sh, w, g, v, 0, false, c, nact);
+                inner(run, sh, w, g, v, 0, false, c, nact <= 8 ? nact : 0);
                 if (run.gsync()) return false;
                 run.phase([&](Thread& th, int t) {
                     query_total(w, g, t, v, nact, c);
@@ -310,11 +334,13 @@ struct Wide {
             if ((nm >> 4) != segY) run.phase([&](Thread&, int t) { TK::load_segrow(sh, t, nm >> 4, c); });
             const bool now = !pref;
             const int nact = TK::active_queries(nm, c);
+            const int cnt0 = nact < 8 ? nact : 8;
             run.phase([&](Thread& th, int t) {
                 TK::stage1_pre(th, sh, t, nm, now, c);
-                query_wave(run, th, sh, t, nm, nact, c);
+                query_wave(run, th, sh, t, nm, 0, cnt0, c);
             });
-            if (nact) run.phase([&](Thread&, int t) { query_slots(sh, w, g, t, nact, c.m); });
+            if (nact) run.phase([&](Thread&, int t) { query_slots(sh, w, g, t, 0, cnt0, c.m); });
+            query_groups(run, sh, w, g, nm, nact, c);
             if (run.gsync()) return false;
             run.phase([&](Thread&, int t) { query_total(w, g, t, nm, nact, c); });
         }

@@ -60,3 +60,29 @@ def test_wide_device_scope_barriers(eng, monkeypatch):
     got = eng.run([(16385, 16386)], [p for _, p in primes], 3)
     want = {i: r for i, _, _, _, r in lines}
     assert [g[0] for g in got] == [want[i] for i, _ in primes]
+
+
+def test_wide_table_with_more_than_eight_queries(eng):
+    # 13 queries on one table of 20000 rows (primes 0..2): the reference's residues for its three
+    # long queries, and the short ones equal to the one-workgroup kernel's own tables of their size
+    recs = load("fast_wide.json")
+    pinned = {}
+    for r in recs:
+        if r["d"] == 3:
+            for i, p, n, M, res in residue_lines(r["stdout"]):
+                pinned[(i, n, M)] = res
+    long_q = [(20000, 40000), (17000, 20000), (16385, 16386)]
+    short_q = [(16000, 16500), (12000, 30000), (8193, 9000), (4000, 4001), (2049, 5000), (1000, 1500), (777, 2000),
+               (200, 400), (64, 65), (37, 80)]
+    primes = cf.prime_range(0, 3)
+    got = eng.run(long_q + short_q, primes, 3)
+    assert eng.stats().log2_length == 16
+    for b in range(3):
+        for j, (n, M) in enumerate(long_q):
+            if (b, n, M) in pinned:
+                assert got[b][j] == pinned[(b, n, M)], (b, n, M)
+    alone = eng.run(short_q, primes, 3)
+    assert eng.stats().log2_length == 14
+    for b in range(3):
+        assert got[b][len(long_q):] == alone[b], b
+    assert got[0][-3:] == [918297158731096, 201776960213191, 67342126954205]  # the reference's, prime 0
