@@ -559,6 +559,43 @@ long cf_max_batch(cf_engine* e, int nmax, int Mmax, size_t J) {
     return (long)(usable_bytes(e) / per);
 }
 
+}  // extern "C"
+
+namespace {
+
+// The queries of primes whose p - 1 lacks the engine's shortest transform, one cubic-algorithm
+// call per query over those primes (cf_check.hip); the stats of the whole call are kept.
+int run_short_primes(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, size_t J, const uint64_t* primes,
+                     const uint8_t* mask, const std::vector<size_t>& idx, uint64_t* res) {
+    const cf_stats before = e->stats;
+    double kernel_ms = 0;
+    int launches = 0;
+    std::vector<uint64_t> ps, out;
+    std::vector<size_t> at;
+    for (size_t j = 0; j < J; ++j) {
+        ps.clear();
+        at.clear();
+        for (size_t b : idx)
+            if (!mask || mask[b * J + j]) ps.push_back(primes[b]), at.push_back(b);
+        if (ps.empty()) continue;
+        out.assign(ps.size(), 0);
+        const int rc = cf_run_algo(e, CF_ALGO_CUBIC, d, model, n[j], M[j], ps.data(), ps.size(), out.data());
+        if (rc) return rc;
+        kernel_ms += e->stats.kernel_ms;
+        launches += e->stats.launches;
+        for (size_t k = 0; k < at.size(); ++k) res[at[k] * J + j] = out[k];
+    }
+    e->stats = before;
+    e->stats.kernel_ms += kernel_ms;
+    e->stats.launches += launches;
+    e->stats.tasks += (int)idx.size();
+    return CF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cf_run_fused(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, size_t J, const uint64_t* primes,
                  size_t np, const uint8_t* mask, uint64_t* res) {
     if (!e) return CF_E_ARGS;
@@ -574,11 +611,22 @@ int cf_run_fused(cf_engine* e, int d, int model, const int32_t* n, const int32_t
 
     // ---- one task per prime that has any query ----
     std::vector<TaskPlan> tasks;
+    std::vector<size_t> short_primes;
     for (size_t b = 0; b < np; ++b) {
         std::vector<int> qi;
+        int nm = 0;
         for (size_t j = 0; j < J; ++j)
-            if (!mask || mask[b * J + j]) qi.push_back((int)j);
+            if (!mask || mask[b * J + j]) qi.push_back((int)j), nm = std::max(nm, (int)n[j]);
         if (qi.empty()) continue;
+        // p - 1 without the engine's shortest transform (2^10) but with the table's own power of
+        // two next_pow2(nm) (a 32-bit NTT prime with 2^8 | p - 1, say): the table has at most 512
+        // rows, and the O(n^3) device algorithm gives its residues
+        const int lg_own = nm > 1 ? 32 - __builtin_clz((unsigned)(nm - 1)) : 1;
+        if (primes[b] < (uint64_t(1) << 50) && engine_log2_length(nm, primes[b]) < 0 &&
+            two_adic_valuation(primes[b] - 1) >= lg_own && nm <= 1024) {
+            short_primes.push_back(b);
+            continue;
+        }
         TaskPlan tp;
         int rc = plan_task(e, primes[b], d, model, qi.size(), [&](size_t k) {
             const int j = qi[k];
@@ -587,7 +635,8 @@ int cf_run_fused(cf_engine* e, int d, int model, const int32_t* n, const int32_t
         if (rc) return rc;
         tasks.push_back(std::move(tp));
     }
-    const int rc = execute(e, tasks, np * J, res, false);
+    int rc = execute(e, tasks, np * J, res, false);
+    if (rc == CF_OK && !short_primes.empty()) rc = run_short_primes(e, d, model, n, M, J, primes, mask, short_primes, res);
     e->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return rc;
 }

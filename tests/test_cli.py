@@ -121,3 +121,16 @@ def test_fail_cli_matches_reference(idx):
             assert gl.split("time:")[0] == wl.split("time:")[0]
         if wl.startswith("N_bad mod"):
             assert gl.split("(")[0] == wl.split("(")[0]
+
+
+@pytest.mark.gpu
+def test_fail_cli_mod_prime_below_the_engine_transform():
+    # 7681 = 15 * 2^9 + 1: next_pow2(200) = 2^8 divides p - 1 (the reference's requirement), the
+    # engine's shortest transform 2^10 does not; the residue is the exact N_bad's
+    from golden_data import fail_case, fail_field
+    exact = int(fail_field(fail_case(["single", "-n", "200", "-M", "400", "-d", "3", "--algo", "ntt", "--exact"])["stdout"],
+                           "N_bad"))
+    r = subprocess.run([cf.CLI_FAIL, "single", "-n", "200", "-M", "400", "-d", "3", "--mod", "7681"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.splitlines()[-1].startswith(f"N_bad mod 7681 = {exact % 7681}  (")

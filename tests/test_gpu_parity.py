@@ -48,7 +48,7 @@ def test_reference_residues_n16384(eng):
     rec = load("fast_16384.json")
     _run_golden(eng, rec)
     st = eng.stats()
-    assert st.log2_length == 14 and st.tasks == 8
+    assert st.log2_length == 14 and st.tasks == rec["count"]
 
 
 @pytest.mark.parametrize("n,M,d,rep,nbad,total", check_instances())
@@ -119,3 +119,16 @@ def test_invalid_arguments_raise(eng):
     assert ei.value.code == cf.CF_E_ARGS
     with pytest.raises(cf.CfError):
         eng.run([(10, 20)], [1000003], 0)
+
+
+def test_primes_below_the_engine_transform(eng):
+    # 7681 = 15 * 2^9 + 1 and 40961 = 5 * 2^13 + 1: next_pow2(n) divides p - 1 for n <= 512 but the
+    # engine's shortest transform 2^10 does not for 7681, so that prime's tables take the O(n^3)
+    # device algorithm; every residue must still be the oracle's
+    qs = [(200, 400), (64, 65), (37, 80), (512, 700)]
+    primes = [7681, 40961] + cf.prime_range(0, 1)
+    got = eng.run(qs, primes, 3, mask=[[1, 1, 1, 0], [1, 1, 1, 1], [1, 1, 1, 1]])
+    for b, p in enumerate(primes):
+        assert [got[b][j] for j in range(3)] == oracle.nbad(3, p, qs[:3]), p
+    assert got[0][3] == 0 and got[1][3] == oracle.nbad(3, 40961, [(512, 700)])[0]
+    assert got[2][:3] == [918297158731096, 201776960213191, 67342126954205]
