@@ -559,6 +559,14 @@ long cf_max_batch(cf_engine* e, int nmax, int Mmax, size_t J) {
     return (long)(usable_bytes(e) / per);
 }
 
+size_t cf_mem_per_prime62(int nmax, int Mmax, size_t J) { return nmax < 1 ? 0 : p62_task_bytes(nmax, Mmax, J); }
+
+long cf_max_batch62(cf_engine* e, int nmax, int Mmax, size_t J) {
+    if (!e || nmax < 1) return 0;
+    (void)hipSetDevice(e->device);
+    return (long)(usable_bytes(e) / p62_task_bytes(nmax, Mmax, J));
+}
+
 }  // extern "C"
 
 namespace {
@@ -611,8 +619,12 @@ int cf_run_fused(cf_engine* e, int d, int model, const int32_t* n, const int32_t
 
     // ---- one task per prime that has any query ----
     std::vector<TaskPlan> tasks;
-    std::vector<size_t> short_primes;
+    std::vector<size_t> short_primes, big_primes;
     for (size_t b = 0; b < np; ++b) {
+        if (primes[b] >= (uint64_t(1) << 50)) {  // beyond the FP64 product: the 64-bit Montgomery kernel
+            big_primes.push_back(b);
+            continue;
+        }
         std::vector<int> qi;
         int nm = 0;
         for (size_t j = 0; j < J; ++j)
@@ -637,6 +649,7 @@ int cf_run_fused(cf_engine* e, int d, int model, const int32_t* n, const int32_t
     }
     int rc = execute(e, tasks, np * J, res, false);
     if (rc == CF_OK && !short_primes.empty()) rc = run_short_primes(e, d, model, n, M, J, primes, mask, short_primes, res);
+    if (rc == CF_OK && !big_primes.empty()) rc = run_p62(e, d, model, n, M, J, primes, big_primes, mask, res);
     e->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return rc;
 }

@@ -66,5 +66,11 @@ int ensure(cf_engine* e, void** buf, size_t* cap, size_t bytes, const char* what
 size_t usable_bytes(cf_engine* e);
 // Runs every task (see cf_engine.cpp); res gets each query's residue at its out index.
 int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* res, bool queue);
+// The queries of primes above 2^50 (up to 2^63), primes[b] for b in idx, on the 64-bit Montgomery
+// kernel (cf_p62_host.cpp): res[b J + j] as cf_run_fused.
+int run_p62(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, size_t J, const uint64_t* primes,
+            const std::vector<size_t>& idx, const uint8_t* mask, uint64_t* res);
+// Device bytes of one prime on that kernel.
+size_t p62_task_bytes(int nmax, int Mmax, size_t J);
 
 }  // namespace cf_int

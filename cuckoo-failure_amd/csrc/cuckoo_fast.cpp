@@ -26,7 +26,13 @@
 
 namespace {
 
-constexpr int kPrimeBits = 50;  // residues modulo primes c 2^20 + 1 below 2^50
+// residues modulo primes c 2^20 + 1 below 2^50, or below 2^62 for the build that replaces the
+// reference's -DFP_RB=64 one (bin/cuckoo_fast62)
+#ifndef CF_PRIME_BITS
+#define CF_PRIME_BITS 50
+#endif
+constexpr int kPrimeBits = CF_PRIME_BITS;
+static_assert(kPrimeBits == 50 || kPrimeBits == 62, "the reference's two prime lists");
 constexpr int kMaxLog2Rows = 20;
 
 // An early end of the program: the exit status and, unless empty, the line for stderr.
@@ -206,10 +212,12 @@ int run_batches(const CommandLine& cl, const QuerySet& qs, Plan& plan) {
     // batches of one shape (all but the last) replay one captured graph; CF_GRAPHS=0 turns it off
     const char* graphs = std::getenv("CF_GRAPHS");
     cf_set_graphs(e, !(graphs && graphs[0] == '0'));
-    autotune_if_asked(e, qs);
+    if (kPrimeBits == 50) autotune_if_asked(e, qs);
     const size_t J = qs.size();
-    const double gb = cf_mem_per_prime(qs.max_n, qs.max_M, J) / 1073741824.0;
-    const long fit = cl.batch > 0 ? cl.batch : cf_max_batch(e, qs.max_n, qs.max_M, J);
+    const bool wide = kPrimeBits > 50;
+    const double gb = (wide ? cf_mem_per_prime62(qs.max_n, qs.max_M, J) : cf_mem_per_prime(qs.max_n, qs.max_M, J)) / 1073741824.0;
+    const long fit = cl.batch > 0 ? cl.batch
+                                  : (wide ? cf_max_batch62(e, qs.max_n, qs.max_M, J) : cf_max_batch(e, qs.max_n, qs.max_M, J));
     if (fit < 1) {
         std::fprintf(stderr, "not enough device memory: %.2f GB per prime\n", gb);
         cf_close(e);
@@ -218,8 +226,8 @@ int run_batches(const CommandLine& cl, const QuerySet& qs, Plan& plan) {
     const size_t per_batch = (size_t)std::min<long>(fit, (long)plan.todo.size());
     char dev[128] = "";
     cf_device_name(e, dev, sizeof dev);
-    std::fprintf(stderr, "[hip %s] exact FP64 residues, %d-bit primes, d=%d, nmax=%d, L=%zu, %.3f GB per prime, batch %zu, %zu primes to do\n",
-                 dev, kPrimeBits, cl.d, qs.max_n, cf_transform_length((size_t)qs.max_n), gb, per_batch, plan.todo.size());
+    std::fprintf(stderr, "[hip %s] %s residues, %d-bit primes, d=%d, nmax=%d, L=%zu, %.3f GB per prime, batch %zu, %zu primes to do\n",
+                 dev, wide ? "64-bit Montgomery" : "exact FP64", kPrimeBits, cl.d, qs.max_n, cf_transform_length((size_t)qs.max_n), gb, per_batch, plan.todo.size());
 
     const cfcli::Clock clock;
     std::vector<uint64_t> bp, res;

@@ -16,6 +16,7 @@ the programs' outputs are kept, as data.
     python tests/golden/make_golden.py --only serving    # the serving table set's 41 queries, 1536 primes
     python tests/golden/make_golden.py --only uneven_big # a sample of the uneven batch with n > 512
     python tests/golden/make_golden.py --only big        # n = 65536: primes 1..3, and 126 queries on 0..1
+    python tests/golden/make_golden.py --only fast62     # the reference's -DFP_RB=64 build (62-bit primes)
 """
 import json
 import os
@@ -418,6 +419,37 @@ def uneven_sample_golden():
     return {"seed": 0, "golden": gold}
 
 
+def fast62_cases():
+    """The reference's -DFP_RB=64 build (62-bit primes c 2^20 + 1 below 2^62, Montgomery R = 2^64):
+    residue lines over several shapes, the whole need() set of (200, 400, 3) and (64, 65, 3) for
+    exact reconstruction, a query file straddling need(), and one prime at n = 16384."""
+    fast62 = os.path.join(REF, "cuckoo_fast62")
+    cases = []
+    for d in (1, 2, 3, 4, 5):
+        for n in (8, 15, 16, 17, 100, 129):
+            cases.append((d, [(n, n + 1), (n, 2 * n + 3)], 0, 2))
+    cases.append((3, [(37, 80), (64, 65), (200, 400), (512, 634)], 0, 3))
+    cases.append((2, [(1024, 1025), (1023, 3000), (255, 256)], 4, 2))
+    cases.append((3, [(4096, 8192)], 0, 1))
+    cases.append((3, [(10, 11), (8, 200), (40, 41), (12, 13)], 0, 12))
+    cases.append((3, [(64, 65)], 0, 60))
+    cases.append((3, [(200, 400)], 0, 97))
+    cases.append((3, [(16384, 32768)], 0, 1))
+
+    def one(c):
+        d, qs, first, count = c
+        with tempfile.TemporaryDirectory() as td:
+            qf = os.path.join(td, "q.txt")
+            with open(qf, "w") as f:
+                f.write("".join(f"{n} {M}\n" for n, M in qs))
+            rc, so, se = run([fast62, str(d), qf, str(first), str(count)])
+        assert rc == 0, se
+        return {"d": d, "queries": qs, "first": first, "count": count, "stdout": so}
+
+    with ThreadPoolExecutor(max_workers=os.cpu_count()) as ex:
+        return list(ex.map(one, cases))
+
+
 def _write(name, data):
     """Writes one fixture file after its data is complete (an interrupted run leaves the old file)."""
     tmp = os.path.join(HERE, name + ".tmp")
@@ -448,6 +480,9 @@ def main():
         return
     if only == "uneven":
         _write("uneven_batch.json", uneven_golden())
+        return
+    if only == "fast62":
+        _write("fast62_cases.json", fast62_cases())
         return
     if only == "separate":
         _write("fail_separate_cases.json", fail_separate_cases())

@@ -86,3 +86,19 @@ def test_wide_table_with_more_than_eight_queries(eng):
     for b in range(3):
         assert got[b][len(long_q):] == alone[b], b
     assert got[0][-3:] == [918297158731096, 201776960213191, 67342126954205]  # the reference's, prime 0
+
+
+@pytest.mark.parametrize("device_scope", [False, True])
+def test_wide_full_launch_of_16_tasks(eng, monkeypatch, device_scope):
+    # 16 primes of (32768, 65536, d = 3): one launch of 16 tasks x 16 workgroups (the whole device),
+    # every residue against the reference's; both group-barrier forms
+    if device_scope:
+        monkeypatch.setenv("CF_WIDE_DEVICE_BARRIERS", "1")
+    rec = load("fast_32768_launch.json")
+    lines = residue_lines(rec["stdout"])
+    assert len(lines) == 16
+    print(f"16-task launch, device-scope barriers: {device_scope}", flush=True)
+    got = eng.run([(32768, 65536)], [p for _, p, _, _, _ in lines], 3)
+    st = eng.stats()
+    assert st.log2_length == 16 and st.tasks == 16 and st.launches == 1
+    assert [g[0] for g in got] == [r for _, _, _, _, r in lines], ("tasks split over XCDs", st.wide_split)
