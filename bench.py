@@ -156,6 +156,8 @@ def bench_uneven(a):
     tables = sum(len(x[4]) for x in spec)
     mm = sum(len(x[4]) * work_modmul(x[2], x[0]) for x in spec)
     eng = cf.Engine()
+    # the work-queue kernel's variant, chosen by the autotuner on a 1/16 sample of this batch
+    tune = eng.autotune_batch(spec, sample_tables=tables // 16)
     eng.run_batch([x[:4] + (x[4][:1],) for x in spec[:8]])  # warm-up: allocation, code load
     t0 = time.perf_counter()
     got = eng.run_batch(spec)
@@ -200,6 +202,7 @@ def bench_uneven(a):
         "plain_per_instance": {"wall_s": round(wall_p, 3), "kernel_ms": round(plain_kernel_ms, 1),
                                "launches": plain_launches, "value": round(tables / wall_p, 2)},
         "speedup_vs_plain": round(wall_p / wall_q, 3),
+        "autotune": tune,
         "cli_per_instance": {"instances": len(sample), "tables": tables_c, "wall_s": round(wall_c, 3),
                              "value": round(tables_c / wall_c, 2),
                              "speedup_of_batch": round((tables / wall_q) / (tables_c / wall_c), 3),

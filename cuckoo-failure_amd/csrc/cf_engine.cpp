@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -256,7 +257,7 @@ int capture(cf_engine* e, const Recorder& R, size_t o_res, size_t nres) {
         const Rec& r = R.recs[k];
         uint64_t* res = (uint64_t*)(sm + o_res);
         if (r.kind == 1)
-            st = launch_queue(r.grid, r.njobs, (const QueueJob*)(sm + r.o_jb), (const TaskDesc*)(sm + r.o_td),
+            st = launch_queue(r.variant, r.grid, r.njobs, (const QueueJob*)(sm + r.o_jb), (const TaskDesc*)(sm + r.o_td),
                               (const QueryDesc*)(sm + r.o_qd), (const int*)(sm + r.o_sl), (double*)e->arena, r.stride, res,
                               (unsigned*)(sm + r.o_nx), e->stream);
         else
@@ -349,8 +350,9 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
         });
         std::vector<QueueJob> jobs;
         size_t stride = 0;
+        const int qv = e->queue_variant, qthreads = queue_variant_info(qv).threads;
         for (size_t k = 0; k < nq;) {
-            const int lg = tasks[k].lg, G = kQueueSlots(lg);
+            const int lg = tasks[k].lg, G = queue_slots(lg, qthreads);
             const std::vector<int> key = shape(tasks[k]);
             size_t j = k + 1;
             while (j < nq && (int)(j - k) < G && shape(tasks[j]) == key) ++j;
@@ -360,7 +362,7 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
         }
         if (nq) {
             stride = (stride + 1023) & ~(size_t)1023;  // every slot stride a multiple of 256 bytes
-            const size_t wg = std::min({jobs.size(), (size_t)std::max(queue_resident(e->device), 1), budget / stride});
+            const size_t wg = std::min({jobs.size(), (size_t)std::max(queue_resident(e->device, qv), 1), budget / stride});
             if (!wg)
                 return fail(e, CF_E_NOMEM, "one job needs " + std::to_string(stride) + " bytes of device memory, " +
                                                std::to_string(budget) + " usable");
@@ -368,14 +370,14 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
             if (rc) return rc;
             describe(e, tasks, 0, nq, true, td, qd, slots);
             if (record) {
-                R.add(1, tasks[0].lg, (int)wg, 0, (int64_t)(stride / 8), td, qd, slots, &jobs);
+                R.add(1, tasks[0].lg, (int)wg, qv, (int64_t)(stride / 8), td, qd, slots, &jobs);
             } else {
                 Uploaded u;
                 rc = upload(e, td, qd, slots, nres, u, &jobs);
                 if (rc) return rc;
                 float ms = 0;
                 rc = timed_launch(e, [&] {
-                    return launch_queue((int)wg, (int)jobs.size(), u.jobs, u.td, u.qd, u.slots, (double*)e->arena,
+                    return launch_queue(qv, (int)wg, (int)jobs.size(), u.jobs, u.td, u.qd, u.slots, (double*)e->arena,
                                         (int64_t)(stride / 8), u.res, u.next, e->stream);
                 }, qd, u, nres, res, ms);
                 if (rc) return rc;
@@ -688,6 +690,66 @@ int cf_run_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t
     const int rc = execute(e, tasks, (size_t)nres, res, true);
     e->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return rc;
+}
+
+int cf_set_batch_variant(cf_engine* e, int variant) {
+    if (!e || variant < 0 || variant >= kQueueVariants) return CF_E_ARGS;
+    e->queue_variant = variant;
+    return CF_OK;
+}
+
+int cf_autotune_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t* primes, size_t sample_tables,
+                      cf_batch_tune* out) {
+    if (!e || !out || (k && (!inst || !primes))) return CF_E_ARGS;
+    memset(out, 0, sizeof *out);
+    e->err.clear();
+    // the sample: every instance with the same share of its primes (at least one), so the mix of
+    // shapes and lengths is the batch's own
+    int64_t total = 0;
+    for (size_t i = 0; i < k; ++i) total += std::max<int64_t>(inst[i].prime_cnt, 0);
+    if (!total) return fail(e, CF_E_ARGS, "the batch has no tables");
+    const double f = sample_tables ? std::min(1.0, (double)sample_tables / (double)total) : 1.0;
+    std::vector<cf_instance> sub;
+    std::vector<uint64_t> sp;
+    for (size_t i = 0; i < k; ++i) {
+        if (inst[i].prime_cnt <= 0) continue;
+        const int64_t c = std::max<int64_t>(1, (int64_t)std::ceil(f * (double)inst[i].prime_cnt));
+        cf_instance x = inst[i];
+        x.prime_off = (int64_t)sp.size();
+        x.prime_cnt = c;
+        sp.insert(sp.end(), primes + inst[i].prime_off, primes + inst[i].prime_off + c);
+        sub.push_back(x);
+    }
+    const int saved = e->queue_variant;
+    std::vector<uint64_t> first, res(sp.size());
+    for (int v = 0; v < kQueueVariants; ++v) {
+        e->queue_variant = v;
+        // code load and first touch of this variant on one instance, then the timed sample
+        int rc = cf_run_batch(e, sub.data(), 1, sp.data(), res.data());
+        if (!rc) rc = cf_run_batch(e, sub.data(), sub.size(), sp.data(), res.data());
+        if (rc) {
+            e->queue_variant = saved;
+            return rc;
+        }
+        const QueueVariant qv = queue_variant_info(v);
+        out->threads[v] = qv.threads;
+        out->per_cu[v] = qv.per_cu;
+        out->resident[v] = queue_resident(e->device, v);
+        out->ms[v] = e->stats.kernel_ms;
+        if (v == 0) first = res;
+        else if (res != first) {
+            e->queue_variant = saved;
+            return fail(e, CF_E_DEVICE, "work-queue variant " + std::to_string(v) + " gave residues different from variant 0");
+        }
+    }
+    int best = 0;
+    for (int v = 1; v < kQueueVariants; ++v)
+        if (out->ms[v] < out->ms[best]) best = v;
+    e->queue_variant = best;
+    out->variants = kQueueVariants;
+    out->chosen = best;
+    out->tables = (int)sp.size();
+    return CF_OK;
 }
 
 int cf_set_variant(cf_engine* e, int log2_length, int variant) {

@@ -24,6 +24,7 @@ struct cf_engine {
     size_t small_cap = 0;
     std::map<int, cf::SlotPlan> plans;  // by nm
     int variant[32] = {};               // task-kernel variant per log2 transform length (cf_autotune)
+    int queue_variant = 0;              // work-queue kernel variant (cf_autotune_batch)
     // graph mode (cf_set_graphs): a call's uploads, launches and downloads are one hipGraph,
     // captured once per launch-plan signature and replayed while the signature repeats
     bool graphs = false;

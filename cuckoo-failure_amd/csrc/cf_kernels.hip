@@ -10,25 +10,24 @@
 namespace cf {
 
 // ---- the work-queue kernel: a persistent grid over jobs of transform lengths 2^10 .. 2^12 ----
-// A job is 1, 2 or 4 tasks (L = 2^12, 2^11, 2^10) that the workgroup runs side by side, each on
-// its own T = L/16 threads (whole waves), LDS slice and arena slice. The host puts only tasks of
-// identical shape (nm, d, model, query sizes) into one job, so their phase sequences are the same
-// and they share the workgroup barriers; a job with fewer tasks than slots repeats its first
-// task in the spare slots and discards those results. Workgroups take jobs from the queue in
-// the host's order (longest first) until it is empty.
-constexpr int kQueueThreads = Plan<kQueueMaxLog2L>::T;
-
+// A job is up to THREADS/256 tasks of L = 2^12, THREADS/128 of 2^11 or THREADS/64 of 2^10 that
+// the workgroup runs side by side, each on its own T = L/16 threads (whole waves), LDS slice and
+// arena slice. The host puts only tasks of identical shape (nm, d, model, query sizes) into one
+// job, so their phase sequences are the same and they share the workgroup barriers; a job with
+// fewer tasks than slots repeats its first task in the spare slots and discards those results.
+// Workgroups take jobs from the queue in the host's order (longest first) until it is empty.
+template <int THREADS>
 union QueueLds {
-    Shared<12> s12;
-    Shared<11> s11[2];
-    Shared<10> s10[4];
+    Shared<12> s12[THREADS / 256];
+    Shared<11> s11[THREADS / 128];
+    Shared<10> s10[THREADS / 64];
 };
 
-template <int LG>
-__device__ __forceinline__ void queue_job(QueueLds& lds, const QueueJob& jb, const TaskDesc* tasks, const QueryDesc* qs,
+template <int LG, int THREADS>
+__device__ __forceinline__ void queue_job(QueueLds<THREADS>& lds, const QueueJob& jb, const TaskDesc* tasks, const QueryDesc* qs,
                                           const int* slots, double* arena, int64_t slot_stride, uint64_t* res) {
-    constexpr int T = Plan<LG>::T, G = kQueueThreads / T;
-    static_assert(G * sizeof(Shared<LG>) <= sizeof(QueueLds), "LDS");
+    constexpr int T = Plan<LG>::T, G = THREADS / T;
+    static_assert(G * sizeof(Shared<LG>) <= sizeof(QueueLds<THREADS>), "LDS");
     const int g = (int)threadIdx.x / T;  // uniform over each wave, since T is a multiple of 64
     Shared<LG>& sh = reinterpret_cast<Shared<LG>*>(&lds)[g];
     const int k = __builtin_amdgcn_readfirstlane(jb.first + (g < jb.cnt ? g : 0));
@@ -40,11 +39,12 @@ __device__ __forceinline__ void queue_job(QueueLds& lds, const QueueJob& jb, con
         for (int j = run.t; j < c.d->q_cnt; j += T) res[qs[c.d->q_off + j].out] = (uint64_t)Task<LG>::result(j, c);
 }
 
-__global__ void __launch_bounds__(kQueueThreads, 2)
+template <int THREADS, int PER_CU>
+__global__ void __launch_bounds__(THREADS, PER_CU)
     k_queue(const QueueJob* __restrict__ jobs, int njobs, const TaskDesc* __restrict__ tasks, const QueryDesc* __restrict__ qs,
             const int* __restrict__ slots, double* __restrict__ mem, int64_t arena_stride, uint64_t* __restrict__ res,
             unsigned* __restrict__ next) {
-    __shared__ QueueLds lds;
+    __shared__ QueueLds<THREADS> lds;
     __shared__ int take;
     double* arena = mem + arena_stride * blockIdx.x;
     for (;;) {
@@ -55,26 +55,51 @@ __global__ void __launch_bounds__(kQueueThreads, 2)
         if (k >= njobs) break;
         const QueueJob jb = jobs[k];
         switch (jb.lg) {
-            case 10: queue_job<10>(lds, jb, tasks, qs, slots, arena, arena_stride / 4, res); break;
-            case 11: queue_job<11>(lds, jb, tasks, qs, slots, arena, arena_stride / 2, res); break;
-            default: queue_job<12>(lds, jb, tasks, qs, slots, arena, arena_stride, res); break;
+            case 10: queue_job<10, THREADS>(lds, jb, tasks, qs, slots, arena, arena_stride / (THREADS / 64), res); break;
+            case 11: queue_job<11, THREADS>(lds, jb, tasks, qs, slots, arena, arena_stride / (THREADS / 128), res); break;
+            default: queue_job<12, THREADS>(lds, jb, tasks, qs, slots, arena, arena_stride / (THREADS / 256), res); break;
         }
         __syncthreads();  // the results are written before the next job reuses the arena
     }
 }
 
-int queue_resident(int device) {
+// variant 0: 256 threads, compiled for 2 workgroups per CU (the built-in choice); 1: the same
+// workgroup at one per CU (the register budget of a whole SIMD, no spills); 2: 512 threads (two
+// 2^12 tasks, four of 2^11, eight of 2^10 side by side) at one per CU
+QueueVariant queue_variant_info(int variant) {
+    switch (variant) {
+        case 1: return {256, 1};
+        case 2: return {512, 1};
+        default: return {256, 2};
+    }
+}
+
+template <int THREADS, int PER_CU>
+int queue_resident_t(int device) {
     hipDeviceProp_t pr;
     if (hipGetDeviceProperties(&pr, device) != hipSuccess) return 0;
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_queue, kQueueThreads, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_queue<THREADS, PER_CU>, THREADS, 0) != hipSuccess) return 0;
     return pr.multiProcessorCount * (per > 0 ? per : 1);
 }
 
-hipError_t launch_queue(int workgroups, int njobs, const QueueJob* jobs, const TaskDesc* tasks, const QueryDesc* qs,
+int queue_resident(int device, int variant) {
+    switch (variant) {
+        case 1: return queue_resident_t<256, 1>(device);
+        case 2: return queue_resident_t<512, 1>(device);
+        default: return queue_resident_t<256, 2>(device);
+    }
+}
+
+hipError_t launch_queue(int variant, int workgroups, int njobs, const QueueJob* jobs, const TaskDesc* tasks, const QueryDesc* qs,
                         const int* slots, double* mem, int64_t arena_stride, uint64_t* res, unsigned* next, hipStream_t stream) {
     if (njobs <= 0 || workgroups <= 0) return hipSuccess;
-    k_queue<<<dim3((unsigned)workgroups), kQueueThreads, 0, stream>>>(jobs, njobs, tasks, qs, slots, mem, arena_stride, res, next);
+    const dim3 grid((unsigned)workgroups);
+    switch (variant) {
+        case 1: k_queue<256, 1><<<grid, 256, 0, stream>>>(jobs, njobs, tasks, qs, slots, mem, arena_stride, res, next); break;
+        case 2: k_queue<512, 1><<<grid, 512, 0, stream>>>(jobs, njobs, tasks, qs, slots, mem, arena_stride, res, next); break;
+        default: k_queue<256, 2><<<grid, 256, 0, stream>>>(jobs, njobs, tasks, qs, slots, mem, arena_stride, res, next); break;
+    }
     return hipGetLastError();
 }
 

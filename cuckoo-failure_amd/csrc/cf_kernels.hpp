@@ -30,17 +30,26 @@ hipError_t launch_variant(int log2L, int variant, int ntasks, const TaskDesc* ta
                           double* mem, uint64_t* res, hipStream_t stream);
 
 // The work-queue kernel: jobs of tasks with log2 L in [kMinLog2L, kQueueMaxLog2L], any mix,
-// ordered as they should start. A job is up to kQueueSlots(lg) consecutive tasks of identical
-// shape that one workgroup runs side by side. `workgroups` persistent workgroups; workgroup w
-// uses the arena at mem + w * arena_stride (doubles), slot g of an lg job its part
-// g * arena_stride / kQueueSlots(lg). *next must be 0 at launch (the queue position).
+// ordered as they should start. A job is up to queue_slots(lg, threads) consecutive tasks of
+// identical shape that one workgroup runs side by side, each on L/16 of its threads.
+// `workgroups` persistent workgroups; workgroup w uses the arena at mem + w * arena_stride
+// (doubles), slot g of an lg job its part g * arena_stride / queue_slots(lg, threads). *next must
+// be 0 at launch (the queue position). Compiled variants (the autotuner of the batched form picks
+// one, cf_autotune_batch; variant 0 is the built-in choice) differ in the workgroup's threads and
+// the workgroups per CU its register budget is compiled for.
 constexpr int kQueueMaxLog2L = 12;
-constexpr int kQueueSlots(int lg) { return 1 << (kQueueMaxLog2L - lg); }
+constexpr int kQueueVariants = 3;
+struct QueueVariant {
+    int threads;  // per workgroup: one 2^12 task per 256
+    int per_cu;   // workgroups per CU the register budget is compiled for
+};
+QueueVariant queue_variant_info(int variant);
+inline int queue_slots(int lg, int threads) { return threads / ((1 << lg) / 16); }
 struct QueueJob {
     int first, cnt, lg, pad_;
 };
-int queue_resident(int device);
-hipError_t launch_queue(int workgroups, int njobs, const QueueJob* jobs, const TaskDesc* tasks, const QueryDesc* qs,
+int queue_resident(int device, int variant = 0);
+hipError_t launch_queue(int variant, int workgroups, int njobs, const QueueJob* jobs, const TaskDesc* tasks, const QueryDesc* qs,
                         const int* slots, double* mem, int64_t arena_stride, uint64_t* res, unsigned* next, hipStream_t stream);
 
 // Tasks with transforms of 2^16, each spread over 16 workgroups that meet at group barriers: one

@@ -54,6 +54,12 @@ class Tune(ctypes.Structure):
                 ("resident", ctypes.c_int * 4), ("ms", ctypes.c_double * 4)]
 
 
+class BatchTune(ctypes.Structure):
+    _fields_ = [("variants", ctypes.c_int), ("chosen", ctypes.c_int), ("tables", ctypes.c_int),
+                ("threads", ctypes.c_int * 4), ("per_cu", ctypes.c_int * 4), ("resident", ctypes.c_int * 4),
+                ("ms", ctypes.c_double * 4)]
+
+
 class ServeStats(ctypes.Structure):
     _fields_ = [("wall_s", ctypes.c_double), ("queries_per_s", ctypes.c_double), ("p50_ms", ctypes.c_double),
                 ("p99_ms", ctypes.c_double), ("mean_batch", ctypes.c_double), ("batches", ctypes.c_long)]
@@ -80,6 +86,13 @@ def lib():
                                       ctypes.c_int, ctypes.POINTER(u64), ctypes.c_size_t, ctypes.POINTER(u64)]
         L.cf_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tune)]
         L.cf_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.cf_set_batch_variant.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.cf_autotune_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Instance), ctypes.c_size_t, ctypes.POINTER(u64),
+                                        ctypes.c_size_t, ctypes.POINTER(BatchTune)]
+        L.cf_max_batch62.restype = ctypes.c_long
+        L.cf_max_batch62.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
+        L.cf_mem_per_prime62.restype = ctypes.c_size_t
+        L.cf_mem_per_prime62.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t]
         L.cf_run_fused.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
                                    ctypes.POINTER(ctypes.c_int32), ctypes.c_size_t, ctypes.POINTER(u64), ctypes.c_size_t,
                                    ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(u64)]
@@ -236,6 +249,33 @@ class Engine:
 
     def set_variant(self, log2_length, variant):
         rc = lib().cf_set_variant(self._h, log2_length, variant)
+        if rc != CF_OK:
+            raise CfError(rc, "bad variant")
+
+    def _instances(self, instances):
+        k = len(instances)
+        inst = (Instance * max(k, 1))()
+        flat = []
+        for i, (d, model, n, M, primes) in enumerate(instances):
+            inst[i] = Instance(d, model, n, M, len(flat), len(primes))
+            flat.extend(primes)
+        return inst, k, flat
+
+    def autotune_batch(self, instances, sample_tables=0):
+        """Times the work-queue kernel's variants on a sample of this batch (instances as for
+        run_batch; about sample_tables tables, 0 = all) and keeps the fastest; returns the table."""
+        inst, k, flat = self._instances(instances)
+        P = (u64 * max(len(flat), 1))(*flat)
+        t = BatchTune()
+        rc = lib().cf_autotune_batch(self._h, inst, k, P, sample_tables, ctypes.byref(t))
+        if rc != CF_OK:
+            raise CfError(rc, lib().cf_last_error(self._h).decode())
+        return {"chosen": t.chosen, "tables": t.tables,
+                "variants": [{"threads": t.threads[v], "workgroups_per_cu": t.per_cu[v], "resident_workgroups": t.resident[v],
+                              "kernel_ms": round(t.ms[v], 2)} for v in range(t.variants)]}
+
+    def set_batch_variant(self, variant):
+        rc = lib().cf_set_batch_variant(self._h, variant)
         if rc != CF_OK:
             raise CfError(rc, "bad variant")
 

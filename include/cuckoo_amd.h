@@ -134,7 +134,7 @@ int cf_run_separate(cf_engine* e, int algo, int n, const int32_t* m, int d, cons
  * n = n_cal, which must lie in the class (L/2, L], or L/2 + 1 for n_cal = 0), checks that all give
  * identical residues, and makes the fastest the engine's choice for that class in later calls.
  * The best variant can depend on n within a class: tune on the sizes that will run. The
- * work-queue kernel of cf_run_batch is not affected. */
+ * work-queue kernel of cf_run_batch has variants of its own (cf_autotune_batch). */
 typedef struct {
     int log2_length;  /* the class */
     int variants;     /* how many variants were timed */
@@ -149,6 +149,28 @@ typedef struct {
 int cf_autotune(cf_engine* e, int log2_length, int n_cal, cf_tune* out);
 /* Selects a class's variant by hand (0 = the built-in choice). */
 int cf_set_variant(cf_engine* e, int log2_length, int variant);
+
+/* The autotuner of the batched form (cf_run_batch; no counterpart in the reference, which runs one
+ * process per instance with one fixed schedule, fast/cuckoo_fast.cu:510-646). The work-queue
+ * kernel is compiled in variants that differ in threads per workgroup (how many tasks of one length
+ * share a workgroup's barriers) and in the workgroups per CU its register budget is compiled for;
+ * variant 0 is the built-in choice. cf_autotune_batch times every variant on a sample of the
+ * caller's own batch (each instance with the same share of its primes, at least one, about
+ * sample_tables tables in all; 0 = the whole batch), requires identical residues from all of
+ * them, and keeps the fastest for this engine's later cf_run_batch calls. */
+typedef struct {
+    int variants;     /* how many were timed */
+    int chosen;       /* the fastest, now used by this engine */
+    int tables;       /* tables in the sample */
+    int threads[4];   /* threads per workgroup */
+    int per_cu[4];    /* workgroups per CU the register budget is compiled for */
+    int resident[4];  /* workgroups resident on the device at once */
+    double ms[4];     /* kernel time of the sample */
+} cf_batch_tune;
+int cf_autotune_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t* primes, size_t sample_tables,
+                      cf_batch_tune* out);
+/* Selects the work-queue variant by hand (0 = the built-in choice). */
+int cf_set_batch_variant(cf_engine* e, int variant);
 
 /* ---- resident tables and serving ---- */
 typedef struct cf_tables cf_tables;

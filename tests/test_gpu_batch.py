@@ -99,3 +99,39 @@ def test_batch_edges(eng):
     with pytest.raises(cf.CfError) as ei:
         eng.run_batch([(3, 0, 70000, 140000, P[:1])])  # transform beyond 2^16
     assert ei.value.code == cf.CF_E_UNSUPPORTED
+
+
+def test_uneven_sample_of_large_instances_vs_reference(eng):
+    # instances with n > 512 (every 5th in order of n, and the three largest): the reference's first
+    # lines verbatim and the CRT-combined exact N_bad by hash, all in one batch
+    try:
+        gold = load("uneven_batch_big.json")["golden"]
+    except OSError:
+        pytest.skip("fixture not generated")
+    insts = [(x["d"], 0, x["n"], x["M"], cf.prime_range(0, x["need"])) for x in gold]
+    print(f"uneven sample: {len(insts)} instances, {sum(len(i[4]) for i in insts)} tables", flush=True)
+    res = eng.run_batch(insts)
+    for x, inst, r in zip(gold, insts, res):
+        for i, p, n, M, want in residue_lines("\n".join(x["head"])):
+            assert inst[4][i] == p and r[i] == want, (x["index"], i)
+        assert _nbad_hash(inst[4], r) == (x["nbad_bits"], x["nbad_sha256"]), x["index"]
+
+
+def test_queue_variants_and_batch_autotuner(eng):
+    # every compiled variant of the work-queue kernel gives the per-instance path's residues; the
+    # tuner times them on a sample of the batch and keeps the fastest
+    P = cf.prime_range(0, 40)
+    spec = [(3, 0, 3000, 7000, P[:24]), (2, 0, 1500, 2000, P[:30]), (4, 1, 700, 900, P[5:37]),
+            (2, 0, 300, 301, P[:40]), (5, 0, 64, 70, P[:11]), (3, 0, 1025, 2000, P[10:20])]
+    want = [[row[0] for row in eng.run([(n, M)], primes, d, model=model)] for d, model, n, M, primes in spec]
+    for v in range(3):
+        eng.set_batch_variant(v)
+        assert eng.run_batch(spec) == want, v
+    t = eng.autotune_batch(spec, sample_tables=60)
+    print(t, flush=True)
+    assert len(t["variants"]) == 3 and 0 <= t["chosen"] < 3 and t["tables"] >= len(spec)
+    assert t["variants"][0]["threads"] == 256 and t["variants"][2]["threads"] == 512
+    ms = [v["kernel_ms"] for v in t["variants"]]
+    assert ms[t["chosen"]] == min(ms)
+    assert eng.run_batch(spec) == want
+    eng.set_batch_variant(0)
