@@ -135,6 +135,19 @@ def serving_leg(a):
             "identical_answers": "served = sequential log2 p_fail for every request of the prefix"}
 
 
+def measured_modmul_peak():
+    """The exact FP64 modular product's peak rate on this GPU (bin/cf_modmul_rate: 8 independent
+    chains per thread, 2048 x 256 threads), or None if the microbenchmark is not built."""
+    import cuckoo_amd as cf
+    exe = os.path.join(os.path.dirname(cf.CLI_FAST), "cf_modmul_rate")
+    if not os.path.exists(exe):
+        return None
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    if r.returncode:
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def profiled_traffic():
     path = os.path.join(REPO, "profiles", "k_tasks14_n16384_traffic.json")
     try:
@@ -409,6 +422,7 @@ def main():
     mm_per_launch = work_modmul(N, D) * tasks / launches
     achieved = mm_per_launch * FLOP_PER_MODMUL / (per_launch_ms * 1e-3) / 1e12
     base = None if a.cpu_baseline == "none" else cpu_baseline(a.cpu_baseline)
+    mpeak = measured_modmul_peak()
     out = {
         "metric": "prime-residues/s at n=16384,d=3 (bit-exact); nbad queries/s + p99 latency",
         "value": round(value, 3),
@@ -428,7 +442,10 @@ def main():
         "roofline": {"bound": "valu", "kernel": "k_tasks<14>", "achieved": round(achieved, 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
                      "traffic": profiled_traffic(), "launch_ms": round(per_launch_ms, 2),
-                     "modmul_per_launch": mm_per_launch},
+                     "modmul_per_launch": mm_per_launch,
+                     "measured_modmul_peak_per_s": mpeak["fp64_exact_modmul_per_s"] if mpeak else None,
+                     "frac_of_measured_modmul_peak": round(mm_per_launch / (per_launch_ms * 1e-3) /
+                                                           mpeak["fp64_exact_modmul_per_s"], 4) if mpeak else None},
         "cpu_baseline": base,
         "bit_exact": {"residues_per_step_equal_to_reference": len(pinned), "steps_checked": a.steps,
                       "reference": "the reference's host build (fast/cuckoo_fast.cu) on primes 0..%d" % (max(pinned) if pinned else -1)},
