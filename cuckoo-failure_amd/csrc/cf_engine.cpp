@@ -545,18 +545,25 @@ int cf_device_name(const cf_engine* e, char* buf, size_t len) {
     return CF_OK;
 }
 
+// Tables beyond the 2^16 transform are not run; their memory is estimated (the two tile pools
+// hold about nm^2 / 4 entries each) without building a slot plan of that size.
+static size_t beyond_engine_bytes(int nmax, int Mmax, size_t J) {
+    const size_t nm = (size_t)nmax, F1 = (size_t)std::max(nmax, Mmax) + 1;
+    return 8 * (nm * nm / 2 + 64 * (nm + 1) + 4 * F1 + 2 * J);
+}
+
 size_t cf_mem_per_prime(int nmax, int Mmax, size_t J) {
     if (nmax < 1) return 0;
-    int lg = kMinLog2L;
-    while ((1 << lg) < nmax) ++lg;
+    if (nmax > (1 << kWideLog2)) return beyond_engine_bytes(nmax, Mmax, J);
+    const int lg = engine_log2_length(nmax, 1 + (uint64_t(1) << 49));
     return task_bytes(plan_slots(nmax), std::max(nmax, Mmax) + 1, (int)J, lg);
 }
 
 long cf_max_batch(cf_engine* e, int nmax, int Mmax, size_t J) {
     if (!e || nmax < 1) return 0;
     (void)hipSetDevice(e->device);
-    int lg = kMinLog2L;
-    while ((1 << lg) < nmax) ++lg;
+    if (nmax > (1 << kWideLog2)) return (long)(usable_bytes(e) / beyond_engine_bytes(nmax, Mmax, J));
+    const int lg = engine_log2_length(nmax, 1 + (uint64_t(1) << 49));
     const size_t per = task_bytes(plan_for(e, nmax), std::max(nmax, Mmax) + 1, (int)J, lg);
     return (long)(usable_bytes(e) / per);
 }

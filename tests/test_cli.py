@@ -134,3 +134,50 @@ def test_fail_cli_mod_prime_below_the_engine_transform():
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert r.stdout.splitlines()[-1].startswith(f"N_bad mod 7681 = {exact % 7681}  (")
+
+
+@pytest.mark.gpu
+def test_fast_cli_not_enough_device_memory_exits_3():
+    # one prime of a table of 2^20 - 1 rows needs terabytes: no batch fits (fast/cuckoo_fast.cu:751-753)
+    rc, so, se, _ = run_fast(3, [(1048575, 2000000)], 0, 1)
+    assert rc == 3 and so == ""
+    assert se.strip().splitlines()[-1].startswith("not enough device memory:"), se
+
+
+@pytest.mark.gpu
+def test_fast_cli_killed_mid_run_then_resumed():
+    # a run cut off after its first batches reached the file (SIGKILL), then the same command again:
+    # every prime of the range ends up in the file once with the uninterrupted run's residue (a
+    # line cut in half by the kill is followed by the resumed output on the same line, as with the
+    # reference; such a line has more than five fields and is not a record)
+    import signal
+    import time
+    queries = [(8192, 16384), (5000, 9000)]
+    with tempfile.TemporaryDirectory() as td:
+        qf, of = os.path.join(td, "q.txt"), os.path.join(td, "out.txt")
+        with open(qf, "w") as f:
+            f.write("".join(f"{n} {M}\n" for n, M in queries))
+        argv = [cf.CLI_FAST, "3", qf, "0", "24", "--batch", "2", "--out", of]
+        p = subprocess.Popen(argv, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        t0 = time.time()
+        while time.time() - t0 < 120 and p.poll() is None:
+            if os.path.exists(of) and open(of).read().count("\n") >= 4:
+                break
+            time.sleep(0.05)
+        killed = p.poll() is None
+        if killed:
+            p.send_signal(signal.SIGKILL)
+        p.wait(timeout=60)
+        print(f"killed mid-run: {killed}; lines before resume: {open(of).read().count(chr(10))}", flush=True)
+        r = subprocess.run(argv, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stderr
+        text = open(of).read()
+        clean = subprocess.run([cf.CLI_FAST, "3", qf, "0", "24"], capture_output=True, text=True, timeout=600)
+        assert clean.returncode == 0
+    want = {(t[0], t[2], t[3]): t[4] for t in (l.split() for l in clean.stdout.splitlines())}
+    recs = [t for t in (l.split() for l in text.splitlines()) if len(t) == 5]
+    got = {}
+    for t in recs:
+        assert want[(t[0], t[2], t[3])] == t[4], t
+        got[(t[0], t[2], t[3])] = t[4]
+    assert got == want
