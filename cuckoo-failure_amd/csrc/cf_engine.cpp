@@ -90,9 +90,9 @@ int plan_task(cf_engine* e, uint64_t p, int d, int model, size_t nq, Q query, Ta
     if (tp.lg < 0)
         return fail(e, CF_E_UNSUPPORTED,
                     "p - 1 of " + std::to_string(p) + " has too few factors of two for n = " + std::to_string(tp.nm));
-    if (tp.lg > kMaxLog2L && tp.lg != kWideLog2)
+    if (tp.lg > kWideMaxLog2 || (tp.lg > kMaxLog2L && tp.lg < kWideLog2))
         return fail(e, CF_E_UNSUPPORTED, "n = " + std::to_string(tp.nm) + " needs a transform longer than 2^" +
-                                             std::to_string(kWideLog2) + ", which this build does not run");
+                                             std::to_string(kWideMaxLog2) + ", which this build does not run");
     tp.bytes = task_bytes(plan_for(e, tp.nm), tp.F1, (int)tp.queries.size(), tp.lg);
     // two convolutions of length L per row, plus the tile traffic
     tp.cost = 2.0 * tp.nm * ((double)(1 << tp.lg) * (tp.lg + 1)) + 8.0 * tp.nm * (double)tp.nm;
@@ -111,7 +111,7 @@ void describe(cf_engine* e, const std::vector<TaskPlan>& tasks, size_t at, size_
         const TaskPlan& tp = tasks[k];
         TaskDesc& t = td[k - at];
         memset(&t, 0, sizeof t);
-        if (tp.lg == kWideLog2) wide_constants(tp.p, t);
+        if (tp.lg >= kWideLog2) wide_constants(tp.p, tp.lg, t);
         else task_constants(tp.p, tp.lg, t);
         const SlotPlan& sp = plan_for(e, tp.nm);
         auto so = slot_off.find(tp.nm);
@@ -331,7 +331,7 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
     float kernel_ms_total = 0;
     // graph mode records the launches and runs them as one graph at the end (not with the
     // cooperative 2^16 launches)
-    const bool record = e->graphs && std::none_of(tasks.begin(), tasks.end(), [](const TaskPlan& t) { return t.lg == kWideLog2; });
+    const bool record = e->graphs && std::none_of(tasks.begin(), tasks.end(), [](const TaskPlan& t) { return t.lg >= kWideLog2; });
     Recorder R;
     std::vector<TaskDesc> td;
     std::vector<QueryDesc> qd;
@@ -407,9 +407,9 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
                                            std::to_string(budget) + " usable");
         size_t cls_end = end;
         while (cls_end < tasks.size() && tasks[cls_end].lg == lg) ++cls_end;
-        const bool wide = lg == kWideLog2;
+        const bool wide = lg >= kWideLog2;
         const int variant = wide ? 0 : e->variant[lg];
-        const size_t cap = wide ? (size_t)std::min(kWideMaxTasks, std::max(wide_resident(e->device), 0) / 16)
+        const size_t cap = wide ? (size_t)std::min(kWideMaxTasks, std::max(wide_resident(lg, e->device), 0) / 16)
                                 : (size_t)std::max(resident_tasks(lg, e->device, variant), 1);
         if (!cap) return fail(e, CF_E_UNSUPPORTED, "the device cannot hold the 16 workgroups of a 2^16 transform");
         const size_t fit = std::min(end - at, wide ? cap : end - at), left = cls_end - at;
@@ -440,7 +440,7 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
         if (rc) return rc;
         float ms = 0;
         rc = timed_launch(e, [&] {
-            if (wide) return launch_wide(nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, u.sync, wide_local(), e->stream);
+            if (wide) return launch_wide(lg, nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, u.sync, wide_local(), e->stream);
             return launch_tasks(lg, nt, u.td, u.qd, u.slots, (double*)e->arena, u.res, e->stream, variant);
         }, qd, u, nres, res, ms);
         if (rc) return rc;
@@ -545,7 +545,7 @@ int cf_device_name(const cf_engine* e, char* buf, size_t len) {
     return CF_OK;
 }
 
-// Tables beyond the 2^16 transform are not run; their memory is estimated (the two tile pools
+// Tables beyond the 2^17 transform are not run; their memory is estimated (the two tile pools
 // hold about nm^2 / 4 entries each) without building a slot plan of that size.
 static size_t beyond_engine_bytes(int nmax, int Mmax, size_t J) {
     const size_t nm = (size_t)nmax, F1 = (size_t)std::max(nmax, Mmax) + 1;
@@ -554,7 +554,7 @@ static size_t beyond_engine_bytes(int nmax, int Mmax, size_t J) {
 
 size_t cf_mem_per_prime(int nmax, int Mmax, size_t J) {
     if (nmax < 1) return 0;
-    if (nmax > (1 << kWideLog2)) return beyond_engine_bytes(nmax, Mmax, J);
+    if (nmax > (1 << kWideMaxLog2)) return beyond_engine_bytes(nmax, Mmax, J);
     const int lg = engine_log2_length(nmax, 1 + (uint64_t(1) << 49));
     return task_bytes(plan_slots(nmax), std::max(nmax, Mmax) + 1, (int)J, lg);
 }
@@ -562,7 +562,7 @@ size_t cf_mem_per_prime(int nmax, int Mmax, size_t J) {
 long cf_max_batch(cf_engine* e, int nmax, int Mmax, size_t J) {
     if (!e || nmax < 1) return 0;
     (void)hipSetDevice(e->device);
-    if (nmax > (1 << kWideLog2)) return (long)(usable_bytes(e) / beyond_engine_bytes(nmax, Mmax, J));
+    if (nmax > (1 << kWideMaxLog2)) return (long)(usable_bytes(e) / beyond_engine_bytes(nmax, Mmax, J));
     const int lg = engine_log2_length(nmax, 1 + (uint64_t(1) << 49));
     const size_t per = task_bytes(plan_for(e, nmax), std::max(nmax, Mmax) + 1, (int)J, lg);
     return (long)(usable_bytes(e) / per);
@@ -855,7 +855,7 @@ int cf_ntt_primes32(int count, int min_pow2, uint32_t* out) {
 
 int cf_engine_log2_length(int nmax, uint64_t p) {
     const int lg = engine_log2_length(nmax, p);
-    return lg > kMaxLog2L && lg != kWideLog2 ? -1 : lg;
+    return lg > kWideMaxLog2 || (lg > kMaxLog2L && lg < kWideLog2) ? -1 : lg;
 }
 
 }  // extern "C"

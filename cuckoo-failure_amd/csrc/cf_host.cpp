@@ -127,11 +127,11 @@ void task_constants(uint64_t p, int log2L, TaskDesc& d) {
     }
 }
 
-// A task spread over workgroups: inner constants of length 4096 (from the same non-residue, so
-// the inner root is the 16th power of the order-2^16 root), 1/L of the whole 2^16 transform.
-void wide_constants(uint64_t p, TaskDesc& d) {
-    task_constants(p, 12, d);
-    const uint64_t L = uint64_t(1) << 16;
+// A task spread over 16 workgroups: inner constants of length L/16 (from the same non-residue, so
+// the inner root is the 16th power of the order-L root), 1/L of the whole transform.
+void wide_constants(uint64_t p, int log2L, TaskDesc& d) {
+    task_constants(p, log2L - 4, d);  // the inner transform's constants
+    const uint64_t L = uint64_t(1) << log2L;
     uint64_t x = 2;
     while (powmod_u64(x, (p - 1) / 2, p) != p - 1) ++x;
     const uint64_t w = powmod_u64(x, (p - 1) / L, p), wi = powmod_u64(w, L - 1, p);
@@ -168,7 +168,7 @@ SlotPlan plan_slots(int nm) {
 
 size_t task_bytes(const SlotPlan& sp, int F1, int q_cnt, int log2L) {
     // a 2^16 task also holds its exchange rows, outer twiddles and per-workgroup sums
-    const int64_t extra = log2L == 16 ? wide_extra_doubles(q_cnt) : 0;
+    const int64_t extra = log2L >= 16 ? wide_extra_doubles(q_cnt, log2L) : 0;
     return 8 * (size_t)(arena_doubles(sp.pool, sp.NS, F1, q_cnt, 1 << log2L) + extra);
 }
 

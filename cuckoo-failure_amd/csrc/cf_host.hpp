@@ -30,8 +30,9 @@ int two_adic_valuation(uint64_t x);
 int engine_log2_length(int nm, uint64_t p);
 // Constants of one prime for the device (roots, 1/L, the DFT16 roots), balanced doubles.
 void task_constants(uint64_t p, int log2L, TaskDesc& d);
-// Constants of a task with transforms of 2^16 spread over 16 workgroups (inner length 4096).
-void wide_constants(uint64_t p, TaskDesc& d);
+// Constants of a task with transforms of 2^log2L (16 or 17) spread over 16 workgroups (inner
+// length 2^(log2L - 4)).
+void wide_constants(uint64_t p, int log2L, TaskDesc& d);
 
 // Static assignment of the suffix-table tiles to pool memory. A tile holds 16 rows x 16 columns
 // (row-major). Tile (Y, X), X < Y, holds rows 16Y..16Y+15 and columns 16X..16X+15; it is written

@@ -103,14 +103,15 @@ hipError_t launch_queue(int variant, int workgroups, int njobs, const QueueJob* 
     return hipGetLastError();
 }
 
-// ---- tasks spread over kWideG workgroups (transforms of 2^16) ----
+// ---- tasks spread over kWideG workgroups (transforms of 2^16 and 2^17) ----
 // Group barrier of the workgroups of one task: each arrives on a monotone counter and waits until
 // all have arrived. The wait is bounded: a waiter that gives up sets the launch's abort flag,
 // which releases every other waiter, and the task ends with an error instead of hanging.
 constexpr long kWideSpinLimit = 1L << 24;
 static_assert(kWideMembers == kWideG, "workgroups per wide task");
 
-struct WideRunner : DevRunner<1, kWideT> {
+template <int T>
+struct WideRunner : DevRunner<1, T> {
     unsigned* cnt;    // this task's arrival counter
     unsigned* abort;  // the launch's abort flag
     int* flag;        // LDS: the outcome for the whole workgroup
@@ -164,16 +165,18 @@ __device__ __forceinline__ unsigned xcc_id() {
 // checks the record, and only when all kWideG share one XCD (and `local` is allowed) do the
 // remaining barriers use the XCD-local form.
 // sync: [2 ntasks] arrival counters (zero at launch), the abort flag, then kWideG XCD ids per task.
-__global__ void __launch_bounds__(kWideT)
+template <int INNER>
+__global__ void __launch_bounds__(Wide<INNER>::T)
     k_wide(const TaskDesc* __restrict__ tasks, int ntasks, const QueryDesc* __restrict__ qs, const int* __restrict__ slots,
            double* __restrict__ mem, uint64_t* __restrict__ res, unsigned* __restrict__ sync, int local) {
-    __shared__ Wide::Sh sh;
+    using W = Wide<INNER>;
+    __shared__ typename W::Sh sh;
     __shared__ int flag;
     const int L = (int)blockIdx.x, xcd = L & 7, i = L >> 3;
     const int task = xcd + 8 * (i / kWideG), g = i % kWideG;
     if (task >= ntasks) return;
     const TaskDesc* d = tasks + task;
-    WideRunner run;
+    WideRunner<W::T> run;
     run.t = (int)threadIdx.x;
     run.cnt = sync + 2 * task;
     run.abort = sync + 2 * ntasks;
@@ -186,28 +189,41 @@ __global__ void __launch_bounds__(kWideT)
         for (int h = 1; h < kWideG; ++h) same &= xcc[h] == xcc[0];
         run.local = same;  // the same data for every member: one decision per task
     }
-    const Ctx c = Wide::ctx(d, qs, slots, mem, g, d->model, d->dd);
-    const WideArena w = wide_arena_of(mem, *d);
-    const bool ok = Wide::run_task(run, sh, c, w, g);
+    const Ctx c = W::ctx(d, qs, slots, mem, g, d->model, d->dd);
+    const WideArena w = wide_arena_of<INNER>(mem, *d);
+    const bool ok = W::run_task(run, sh, c, w, g);
     if (ok && g == 0)
-        for (int j = threadIdx.x; j < d->q_cnt; j += kWideT) res[qs[d->q_off + j].out] = (uint64_t)Wide::TK::result(j, c);
+        for (int j = threadIdx.x; j < d->q_cnt; j += W::T) res[qs[d->q_off + j].out] = (uint64_t)W::TK::result(j, c);
 }
 
-int wide_resident(int device) {
+template <int INNER>
+int wide_resident_t(int device) {
     hipDeviceProp_t pr;
     if (hipGetDeviceProperties(&pr, device) != hipSuccess) return 0;
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_wide, kWideT, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_wide<INNER>, Wide<INNER>::T, 0) != hipSuccess) return 0;
     return pr.multiProcessorCount * per;
 }
 
-hipError_t launch_wide(int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem, uint64_t* res,
-                       unsigned* sync, int local, hipStream_t stream) {
-    if (ntasks <= 0) return hipSuccess;
+int wide_resident(int log2L, int device) {
+    return log2L == 17 ? wide_resident_t<13>(device) : log2L == 16 ? wide_resident_t<12>(device) : 0;
+}
+
+template <int INNER>
+hipError_t launch_wide_t(int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem,
+                         uint64_t* res, unsigned* sync, int local, hipStream_t stream) {
     const unsigned grid = (unsigned)(((ntasks + 7) / 8) * 8 * kWideG);
     void* args[] = {(void*)&tasks, (void*)&ntasks, (void*)&qs, (void*)&slots, (void*)&mem, (void*)&res, (void*)&sync,
                     (void*)&local};
-    return hipLaunchCooperativeKernel((const void*)k_wide, dim3(grid), dim3(kWideT), args, 0, stream);
+    return hipLaunchCooperativeKernel((const void*)k_wide<INNER>, dim3(grid), dim3(Wide<INNER>::T), args, 0, stream);
+}
+
+hipError_t launch_wide(int log2L, int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem,
+                       uint64_t* res, unsigned* sync, int local, hipStream_t stream) {
+    if (ntasks <= 0) return hipSuccess;
+    if (log2L == 16) return launch_wide_t<12>(ntasks, tasks, qs, slots, mem, res, sync, local, stream);
+    if (log2L == 17) return launch_wide_t<13>(ntasks, tasks, qs, slots, mem, res, sync, local, stream);
+    return hipErrorInvalidValue;
 }
 
 // Workgroups of the log2L kernel that the device runs at the same time (CUs x blocks per CU).

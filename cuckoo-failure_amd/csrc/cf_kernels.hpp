@@ -52,16 +52,17 @@ int queue_resident(int device, int variant = 0);
 hipError_t launch_queue(int variant, int workgroups, int njobs, const QueueJob* jobs, const TaskDesc* tasks, const QueryDesc* qs,
                         const int* slots, double* mem, int64_t arena_stride, uint64_t* res, unsigned* next, hipStream_t stream);
 
-// Tasks with transforms of 2^16, each spread over 16 workgroups that meet at group barriers: one
-// cooperative launch of ntasks tasks (all workgroups resident at once). sync holds
+// Tasks with transforms of 2^16 or 2^17 (log2L), each spread over 16 workgroups (of 256 or 512
+// threads) that meet at group barriers: one cooperative launch of ntasks tasks (all workgroups
+// resident at once). sync holds
 // wide_sync_words(ntasks) zeros at launch; afterwards sync[2 ntasks] != 0 means a barrier timed
 // out (results invalid), and sync[2 ntasks + 2 + 16 k + g] is the XCD that workgroup g of task k
 // ran on. local != 0 lets a task whose workgroups share one XCD use XCD-local barriers (its L2 is
 // then the common cache level).
-constexpr int kWideLog2 = 16, kWideMaxTasks = 16, kWideMembers = 16;
+constexpr int kWideLog2 = 16, kWideMaxLog2 = 17, kWideMaxTasks = 16, kWideMembers = 16;
 constexpr size_t wide_sync_words(int ntasks) { return 2 * (size_t)ntasks + 2 + kWideMembers * (size_t)ntasks; }
-int wide_resident(int device);  // workgroups of the wide kernel the device holds at once
-hipError_t launch_wide(int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem, uint64_t* res,
+int wide_resident(int log2L, int device);  // workgroups of the wide kernel the device holds at once
+hipError_t launch_wide(int log2L, int ntasks, const TaskDesc* tasks, const QueryDesc* qs, const int* slots, double* mem, uint64_t* res,
                        unsigned* sync, int local, hipStream_t stream);
 
 }  // namespace cf

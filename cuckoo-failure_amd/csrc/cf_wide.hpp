@@ -1,15 +1,16 @@
-// cf_wide.hpp -- one prime on kWideG workgroups: the suffix-table recurrence with transforms of
-// length L = 2^16, which do not fit one CU's LDS.
+// cf_wide.hpp -- one prime on kWideG = 16 workgroups: the suffix-table recurrence with transforms
+// of length L = 16 N2 (N2 = 2^INNER: L = 2^16 with INNER = 12, 2^17 with INNER = 13), which do not
+// fit one CU's LDS.
 //
-// The transform is split four-step: entry e = 4096 n1 + n2 (n1 < 16, n2 < 4096). Workgroup g owns
-// the columns n2 = 256 g + t (t < 256): its thread t holds the 16 entries 4096 k + 256 g + t in
-// registers, which is also the element map of every stage function (Ctx::es = 4096,
-// Ctx::eb = 256 g). A convolution is then
-//   outer: DFT16 over n1 in registers, twiddle w^(n2 k1) (w of order 2^16), store row k1 of the
-//          exchange buffer (16 x 4096 doubles in HBM);
+// The transform is split four-step: entry e = N2 n1 + n2 (n1 < 16, n2 < N2). Workgroup g owns the
+// columns n2 = T g + t (t < T = N2 / 16): its thread t holds the 16 entries N2 k + T g + t in
+// registers, which is also the element map of every stage function (Ctx::es = N2, Ctx::eb = T g).
+// A convolution is then
+//   outer: DFT16 over n1 in registers, twiddle w^(n2 k1) (w of order L), store row k1 of the
+//          exchange buffer (16 x N2 doubles in HBM);
 //   group barrier;
-//   inner: workgroup g loads row k1 = g and runs the one-workgroup engine of length 4096
-//          (Engine<12>: rounds in LDS, pointwise product with its slice of the kernel spectrum,
+//   inner: workgroup g loads row k1 = g and runs the one-workgroup engine of length N2
+//          (Engine<INNER>: rounds in LDS, pointwise product with its slice of the kernel spectrum,
 //          inverse rounds), storing the row back;
 //   group barrier;
 //   outer inverse: load the thread's column, twiddle w^-(n2 k1), inverse DFT16.
@@ -25,37 +26,37 @@
 
 namespace cf {
 
-constexpr int kWideG = 16;            // workgroups per prime
-constexpr int kWideLog2L = 16;        // L = 2^16
-constexpr int kWideInner = 12;        // inner transform length 2^12 = 4096
-constexpr int kWideT = 256;           // threads per workgroup
+constexpr int kWideG = 16;  // workgroups per prime
 static_assert(kWideG == 16, "wide_extra_doubles() assumes 16 workgroups per prime");
 
-// Extra arena of a wide task, after the one-workgroup layout of length 2^16 (doubles).
+// Extra arena of a wide task, after the one-workgroup layout of length L = 16 N2 (doubles).
 struct WideArena {
-    double* exch;   // 16 x 4096 exchange rows
-    double* twout;  // per workgroup 2048: w^(n2 2^k) (k < 4) then w^-(n2 2^k), at k 256 + t
+    double* exch;   // 16 x N2 exchange rows
+    double* twout;  // per workgroup 8 T: w^(n2 2^k) (k < 4) then w^-(n2 2^k), at k T + t
     double* qpart;  // [g][qs] per-workgroup query sums
     double* hand;   // [0]: the hand-over entry when it crosses workgroups
     int qs;         // query slots per workgroup: max(8, the task's query count)
 };
+template <int INNER>
 CF_HD WideArena wide_arena_of(double* base, const TaskDesc& t) {
+    constexpr int64_t N2 = int64_t(1) << INNER;
     WideArena w;
-    w.exch = base + t.arena + arena_doubles(t.pool, t.NS, t.F1, t.q_cnt, 1 << kWideLog2L);
-    w.twout = w.exch + 65536;
-    w.qpart = w.twout + 32768;
+    w.exch = base + t.arena + arena_doubles(t.pool, t.NS, t.F1, t.q_cnt, (int)(16 * N2));
+    w.twout = w.exch + 16 * N2;
+    w.qpart = w.twout + 8 * N2;
     w.qs = wide_qslots(t.q_cnt);
     w.hand = w.qpart + (int64_t)kWideG * w.qs;
     return w;
 }
 
+template <int INNER>
 struct Wide {
-    using TK = Task<kWideInner, kWideLog2L>;  // stage functions (wide element map), LDS layout
-    using Sh = Shared<kWideInner, kWideLog2L>;
-    using E = Engine<kWideInner>; // the inner transform
-    using P = Plan<kWideInner>;
-    static constexpr int T = kWideT, G = kWideG, NW = T / 64, N2 = 4096;
-    static_assert(P::T == T, "inner plan");
+    static constexpr int LOG2L = INNER + 4;
+    using TK = Task<INNER, LOG2L>;  // stage functions (wide element map), LDS layout
+    using Sh = Shared<INNER, LOG2L>;
+    using E = Engine<INNER>;        // the inner transform
+    using P = Plan<INNER>;
+    static constexpr int T = P::T, G = kWideG, NW = T / 64, N2 = 1 << INNER;
 
     // The context of workgroup g: the wide element map and its slice of the spectrum.
     static CF_HD Ctx ctx(const TaskDesc* d, const QueryDesc* qs, const int* slots, double* mem, int g, int model, int dd) {
@@ -71,26 +72,26 @@ struct Wide {
         c.dd = dd;
         c.v0 = model ? 1 : dd;
         c.es = N2;
-        c.eb = 256 * g;
-        c.L = 1 << kWideLog2L;
+        c.eb = T * g;
+        c.L = 1 << LOG2L;
         return c;
     }
 
     // ---- the outer step ----
     static CF_HD void outer_twiddles(const WideArena& w, int g, int t, const TaskDesc& d, const Mod& m) {
-        double* tw = w.twout + 2048 * g;
-        const uint64_t n2 = (uint64_t)(256 * g + t);
+        double* tw = w.twout + 8 * T * g;
+        const uint64_t n2 = (uint64_t)(T * g + t);
         double f = powmod(d.wroot, n2, m), i = powmod(d.wrooti, n2, m);
         for (int k = 0; k < 4; ++k) {
-            tw[k * 256 + t] = f, tw[1024 + k * 256 + t] = i;
+            tw[k * T + t] = f, tw[4 * T + k * T + t] = i;
             f = reduce(mulred(f, f, m), m), i = reduce(mulred(i, i, m), m);
         }
     }
     // this thread's w^(n2 2^k), k < 4 (dir 0), or w^-(n2 2^k) (dir 1)
     static CF_HD void outer_bases(double (&b4)[4], const WideArena& w, int g, int t, int dir) {
-        const double* src = w.twout + 2048 * g + 1024 * dir + t;
+        const double* src = w.twout + 8 * T * g + 4 * T * dir + t;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) b4[k] = src[k * 256];
+        for (int k = 0; k < 4; ++k) b4[k] = src[k * T];
     }
     static CF_HD void outer_fwd(Thread& th, const WideArena& w, int g, int t, const TaskDesc& d, const Mod& m) {
         double b4[4];
@@ -99,25 +100,25 @@ struct Wide {
         dft_groups<16>(th.x, d.W, m);
         twiddle_chain(th.x, b4, m);
 #pragma unroll
-        for (int k = 0; k < 16; ++k) w.exch[k * N2 + 256 * g + t] = th.x[k];
+        for (int k = 0; k < 16; ++k) w.exch[k * N2 + T * g + t] = th.x[k];
     }
     static CF_HD void outer_inv(Thread& th, const WideArena& w, int g, int t, const TaskDesc& d, const Mod& m) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) th.x[k] = w.exch[k * N2 + 256 * g + t];
+        for (int k = 0; k < 16; ++k) th.x[k] = w.exch[k * N2 + T * g + t];
         double b4[4];
         outer_bases(b4, w, g, t, 1);
         twiddle_chain(th.x, b4, m);
         reduce16(th.x, m);
         dft_groups<16>(th.x, d.Wi, m);
     }
-    // inner row g: exchange -> registers (entries 256 k + t) and back
+    // inner row g: exchange -> registers (entries T k + t) and back
     static CF_HD void inner_load(Thread& th, const WideArena& w, int g, int t) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) th.x[k] = w.exch[g * N2 + 256 * k + t];
+        for (int k = 0; k < 16; ++k) th.x[k] = w.exch[(int64_t)g * N2 + T * k + t];
     }
     static CF_HD void inner_store(const Thread& th, const WideArena& w, int g, int t) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) w.exch[g * N2 + 256 * k + t] = th.x[k];
+        for (int k = 0; k < 16; ++k) w.exch[(int64_t)g * N2 + T * k + t] = th.x[k];
     }
 
     // The inner part of a convolution (between the two group barriers). The factorials of the
@@ -167,7 +168,7 @@ struct Wide {
     static CF_HD void query_slots(const Sh& sh, const WideArena& w, int g, int t, int j0, int cnt, const Mod& m) {
         if (t < cnt) {
             double s = 0.0;
-            for (int wv = 0; wv < NW; ++wv) s += sh.red[wv * 8 + t];  // 4 values below p/2 + 4
+            for (int wv = 0; wv < NW; ++wv) s += sh.red[wv * 8 + t];  // up to 8 values below p/2 + 4
             w.qpart[(int64_t)g * w.qs + j0 + t] = reduce(s, m);
         }
     }
@@ -309,10 +310,10 @@ struct Wide {
             if (!s1 && !(s2 && v > c.v0) && run.gsync()) return false;
             // the hand-over entry (v+1, v) was made by the owner of entry v+1; when the owner of
             // entry v (which uses it next step) is another workgroup, it goes through memory
-            const bool cross = ((v + 1) & 255) == 0;
+            const bool cross = ((v + 1) & (T - 1)) == 0;
             if (cross) {
                 run.phase([&](Thread&, int t) {
-                    if (((v + 1) & (N2 - 1)) / 256 == g && t == 0) w.hand[0] = sh.scal[1];
+                    if (((v + 1) & (N2 - 1)) / T == g && t == 0) w.hand[0] = sh.scal[1];
                 });
                 if (run.gsync()) return false;
                 run.phase([&](Thread&, int t) {

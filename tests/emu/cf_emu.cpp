@@ -12,7 +12,7 @@
 // prints one line per query, "n M residue", in the order given, then a line
 // "products N max_ratio R" (the largest |product result| / p seen) and exits 0; a failed check
 // prints "EXACTNESS ..." and exits 3. --lg picks the transform length (default: the engine's
-// choice for the largest n; any L >= that n is valid, 16 selects the 16-workgroup kernel),
+// choice for the largest n; any L >= that n is valid, 16 and 17 select the 16-workgroup kernel),
 // --defer -1/0 the tile-flush variant of the one-workgroup kernel, --reverse runs the threads of
 // every phase in descending order (results must not change: a phase may not read what another
 // thread of the same phase writes).
@@ -122,7 +122,8 @@ struct HostBarrier {
     }
 };
 
-struct WideEmuRunner : EmuRunner<kWideT> {
+template <int T>
+struct WideEmuRunner : EmuRunner<T> {
     HostBarrier* bar = nullptr;
     bool gsync() {
         bar->wait();
@@ -145,7 +146,7 @@ void describe(Job& j) {
     j.sp = plan_slots(j.nm);
     TaskDesc& t = j.td;
     std::memset(&t, 0, sizeof t);
-    if (j.lg == 16) wide_constants(j.p, t);
+    if (j.lg >= 16) wide_constants(j.p, j.lg, t);
     else task_constants(j.p, j.lg, t);
     t.arena = 0;
     t.nm = j.nm;
@@ -188,19 +189,21 @@ void run_one_workgroup(Job& j) {
     for (size_t k = 0; k < j.q.size(); ++k) j.res[(size_t)j.q[k].out] = (uint64_t)TK::result((int)k, c);
 }
 
+template <int INNER>
 void run_wide(Job& j) {
+    using W = Wide<INNER>;
     HostBarrier bar(kWideG);
     std::vector<std::thread> ws;
     for (int g = 0; g < kWideG; ++g)
         ws.emplace_back([&, g] {
-            auto run = std::make_unique<WideEmuRunner>();
+            auto run = std::make_unique<WideEmuRunner<W::T>>();
             run->bar = &bar;
-            auto sh = std::make_unique<Wide::Sh>();
-            const Ctx c = Wide::ctx(&j.td, j.q.data(), j.sp.segs.data(), j.mem.data(), g, j.model, j.d);
-            const WideArena w = wide_arena_of(j.mem.data(), j.td);
-            Wide::run_task(*run, *sh, c, w, g);
+            auto sh = std::make_unique<typename W::Sh>();
+            const Ctx c = W::ctx(&j.td, j.q.data(), j.sp.segs.data(), j.mem.data(), g, j.model, j.d);
+            const WideArena w = wide_arena_of<INNER>(j.mem.data(), j.td);
+            W::run_task(*run, *sh, c, w, g);
             if (g == 0)
-                for (size_t k = 0; k < j.q.size(); ++k) j.res[(size_t)j.q[k].out] = (uint64_t)Wide::TK::result((int)k, c);
+                for (size_t k = 0; k < j.q.size(); ++k) j.res[(size_t)j.q[k].out] = (uint64_t)W::TK::result((int)k, c);
         });
     for (auto& t : ws) t.join();
 }
@@ -261,12 +264,14 @@ int main(int argc, char** argv) {
         return 2;
     }
     if (!j.lg) j.lg = need_lg;
-    if (j.lg < need_lg || (j.lg > 14 && j.lg != 16) || two_adic_valuation(j.p - 1) < j.lg) {
+    if (j.lg < need_lg || (j.lg > 14 && j.lg != 16 && j.lg != 17) || two_adic_valuation(j.p - 1) < j.lg) {
         std::fprintf(stderr, "transform 2^%d cannot run n = %d modulo %llu\n", j.lg, j.nm, (unsigned long long)j.p);
         return 2;
     }
     describe(j);
-    const bool ok = j.lg == 16 ? (run_wide(j), true) : (j.defer == 0 ? dispatch<0>(j) : dispatch<-1>(j));
+    const bool ok = j.lg == 16   ? (run_wide<12>(j), true)
+                    : j.lg == 17 ? (run_wide<13>(j), true)
+                                 : (j.defer == 0 ? dispatch<0>(j) : dispatch<-1>(j));
     if (!ok) return 2;
     for (size_t k = 0; k < given.size(); ++k) std::printf("%d %d %llu\n", given[k].n, given[k].M, (unsigned long long)j.res[k]);
     std::printf("products %lld max_ratio %.6f\n", (long long)g_products.load(), g_max_ratio);

@@ -17,6 +17,7 @@ the programs' outputs are kept, as data.
     python tests/golden/make_golden.py --only uneven_big # a sample of the uneven batch with n > 512
     python tests/golden/make_golden.py --only big        # n = 65536: primes 1..3, and 126 queries on 0..1
     python tests/golden/make_golden.py --only fast62     # the reference's -DFP_RB=64 build (62-bit primes)
+    python tests/golden/make_golden.py --only big17      # n = 70000 (transforms of 2^17), prime 0 (~39 GB, ~4 min)
 """
 import json
 import os
@@ -480,6 +481,10 @@ def main():
         return
     if only == "uneven":
         _write("uneven_batch.json", uneven_golden())
+        return
+    if only == "big17":
+        _write("fast_70000.json", {"d": 3, "queries": [[70000, 140000], [66000, 70001]], "first": 0, "count": 1,
+                                   "stdout": cached_run(3, [(70000, 140000), (66000, 70001)], 0, 1)})
         return
     if only == "fast62":
         _write("fast62_cases.json", fast62_cases())

@@ -89,16 +89,16 @@ int main() {
             int lg;
             in >> p >> lg;
             cf::TaskDesc t{};
-            if (lg == 16) cf::wide_constants(p, t);
+            if (lg >= 16) cf::wide_constants(p, lg, t);
             else cf::task_constants(p, lg, t);
             // a root of order exactly 2^k has root^(2^(k-1)) = -1: the transform's root (2^lg; the
-            // inner 2^12 and the outer 2^16 of a 16-workgroup task)
+            // inner 2^(lg-4) and the outer 2^lg of a 16-workgroup task)
             auto order_is = [&](double root, int k) {
                 uint64_t r = (uint64_t)(root < 0 ? root + (double)p : root);
                 for (int i = 0; i < k - 1; ++i) r = cf::mulmod_u64(r, r, p);
                 return r == p - 1;
             };
-            const bool ok = lg == 16 ? order_is(t.root, 12) && order_is(t.wroot, 16) : order_is(t.root, lg);
+            const bool ok = lg >= 16 ? order_is(t.root, lg - 4) && order_is(t.wroot, lg) : order_is(t.root, lg);
             std::printf("%d\n", ok ? 1 : 0);
         } else if (!cmd.empty()) {
             std::printf("? %s\n", cmd.c_str());

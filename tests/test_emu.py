@@ -83,3 +83,11 @@ def test_sixteen_workgroup_kernel_with_many_queries(emu):
     assert got == want
     got, _ = run_emu(emu, 3, 0, P0, qs[:9], lg=16, reverse=True)
     assert got == want[:9]
+
+
+@pytest.mark.slow
+def test_sixteen_workgroup_kernel_of_2_17(emu):
+    # the 2^17 form (16 workgroups of 512 threads, inner transforms of 2^13) on a short table
+    qs = [(300, 900), (300, 301), (200, 400), (64, 65), (20, 40), (9, 10), (250, 251), (100, 800), (90, 91), (30, 31)]
+    got, ratio = run_emu(emu, 3, 0, P0, qs, lg=17)
+    assert got == oracle.nbad(3, P0, qs) and ratio < 2.0

@@ -102,3 +102,30 @@ def test_wide_full_launch_of_16_tasks(eng, monkeypatch, device_scope):
     st = eng.stats()
     assert st.log2_length == 16 and st.tasks == 16 and st.launches == 1
     assert [g[0] for g in got] == [r for _, _, _, _, r in lines], ("tasks split over XCDs", st.wide_split)
+
+
+def test_wide_2_17_against_2_16_and_reference(eng):
+    # tables of more than 65536 rows take transforms of 2^17 (16 workgroups of 512 threads); their
+    # shorter queries must equal the same queries answered from a 2^16 table, and (32768, 65536)
+    # the reference's residues
+    P = cf.prime_range(0, 2)
+    qs17 = [(65537, 70000), (60000, 100000), (32768, 65536)]
+    got = eng.run(qs17, P, 3)
+    assert eng.stats().log2_length == 17
+    short = eng.run(qs17[1:], P, 3)
+    assert eng.stats().log2_length == 16
+    for b in range(2):
+        assert got[b][1:] == short[b], b
+    assert [got[0][2], got[1][2]] == [1051397482096400, 367355341181042]
+
+
+def test_wide_2_17_reference_prime0(eng):
+    try:
+        rec = load("fast_70000.json")
+    except OSError:
+        pytest.skip("fixture not generated")
+    lines = residue_lines(rec["stdout"])
+    qs = [tuple(q) for q in rec["queries"]]
+    got = eng.run(qs, [lines[0][1]], 3)
+    assert eng.stats().log2_length == 17
+    assert got[0] == [r for _, _, _, _, r in lines]

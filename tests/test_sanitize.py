@@ -76,7 +76,7 @@ def test_prime_lists_need_slots_constants(built):
     cmds = ["primes 50 0 6", "primes 62 3 4", "need 16384 32768 3 50", "need 200 400 3 62", "need 65536 131072 3 50"]
     cmds += [f"slots {nm}" for nm in (8, 17, 255, 256, 257, 4096, 16384, 40000, 65536)]
     P = oracle.prime_range(0, 2)
-    cmds += [f"consts {P[0]} {lg}" for lg in (10, 11, 12, 13, 14, 16)] + [f"consts {P[1]} 16"]
+    cmds += [f"consts {P[0]} {lg}" for lg in (10, 11, 12, 13, 14, 16, 17)] + [f"consts {P[1]} 16"]
     out = host(built, cmds)
     assert [int(x) for x in out[0].split()] == oracle.prime_range(0, 6)
     assert [int(x) for x in out[1].split()] == oracle.prime_range(3, 4, 62)
@@ -85,7 +85,7 @@ def test_prime_lists_need_slots_constants(built):
     for line in out[5:14]:
         pool, nseg, nb, segs = (int(v) for v in line.split())
         assert pool > 0 and nseg > 0 and segs == nb * nseg
-    assert out[14:] == ["1"] * 7
+    assert out[14:] == ["1"] * 8
 
 
 @pytest.mark.parametrize("idx", range(7))
