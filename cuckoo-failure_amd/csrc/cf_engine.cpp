@@ -323,6 +323,18 @@ int wide_local() {
     return !(v && v[0] == '1');
 }
 
+// Tasks per launch of the 16-workgroup kernel: as many as the device keeps resident, at most
+// kWideMaxTasks (one workgroup per CU: two per CU, where registers and LDS would allow them,
+// measured slower); CF_WIDE_TASKS=k caps it lower (for measurements).
+size_t wide_task_cap(int lg, int device) {
+    size_t cap = (size_t)std::min(kWideMaxTasks, std::max(wide_resident(lg, device), 0) / kWideMembers);
+    if (const char* v = getenv("CF_WIDE_TASKS")) {
+        const int k = atoi(v);
+        if (k > 0) cap = std::min(cap, (size_t)k);
+    }
+    return cap;
+}
+
 // Runs every task. With queue, the tasks whose transform fits the work-queue kernel go to one
 // persistent launch (longest first); the rest, and everything without queue, run as launches
 // of one transform length each, packed into device memory.
@@ -409,7 +421,7 @@ int execute(cf_engine* e, std::vector<TaskPlan>& tasks, size_t nres, uint64_t* r
         while (cls_end < tasks.size() && tasks[cls_end].lg == lg) ++cls_end;
         const bool wide = lg >= kWideLog2;
         const int variant = wide ? 0 : e->variant[lg];
-        const size_t cap = wide ? (size_t)std::min(kWideMaxTasks, std::max(wide_resident(lg, e->device), 0) / 16)
+        const size_t cap = wide ? wide_task_cap(lg, e->device)
                                 : (size_t)std::max(resident_tasks(lg, e->device, variant), 1);
         if (!cap) return fail(e, CF_E_UNSUPPORTED, "the device cannot hold the 16 workgroups of a 2^16 transform");
         const size_t fit = std::min(end - at, wide ? cap : end - at), left = cls_end - at;
