@@ -108,6 +108,9 @@ hipError_t launch_queue(int variant, int workgroups, int njobs, const QueueJob* 
 // all have arrived. The wait is bounded: a waiter that gives up sets the launch's abort flag,
 // which releases every other waiter, and the task ends with an error instead of hanging.
 constexpr long kWideSpinLimit = 1L << 24;
+#ifndef CF_WIDE_POLL_SLEEP
+#define CF_WIDE_POLL_SLEEP 2  // s_sleep between polls of the arrival counter (units of 64 clocks)
+#endif
 static_assert(kWideMembers == kWideG, "workgroups per wide task");
 
 template <int T>
@@ -140,7 +143,7 @@ struct WideRunner : DevRunner<1, T> {
                     bad = 1;
                     break;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(CF_WIDE_POLL_SLEEP);
             }
             // the other workgroups' writes, not stale cache lines: within one XCD only this CU's
             // L1 can be stale (the L2 is shared); otherwise the device-scope fence
