@@ -16,9 +16,9 @@
 //                      P2[p][v] = (g^v/v! - p c[p mod L]) g^(p-v)/v! / (p-v-1)!^2,  v < p <= nm
 //   queries:           tot_j <- tot_j Q_j + n_j!/((n_j-v)! (M_j-v)!) sum_w a[w] / (M_j - w)   (n_j >= v)
 // with k[t] = 1/t (1 <= t < nm) and the result N_bad = M_j! tot_j. One workgroup runs one prime:
-// the cyclic convolution of length L = max(16, next_pow2(nm)) <= 2^14 in LDS (radix-2 DIF forward,
-// natural in and bit-reversed out; pointwise product with the kernel's spectrum, made by the same
-// forward and scaled by 1/L; radix-2 DIT inverse back to natural order). The tables are packed
+// the cyclic convolution of length L = max(16, next_pow2(nm)) <= 2^14 in LDS (DIF forward, natural
+// in and bit-reversed out; pointwise product with the kernel's spectrum, made by the same forward
+// and scaled by 1/L; DIT inverse back to natural order; two radix-2 levels per LDS pass). The tables are packed
 // triangles in HBM (P2 row p at p(p-1)/2, S2 row e at e(e-1)/2 with entry q at q-1); the 16
 // columns of the current column block go to column-major staging first and are copied into the
 // rows when the block ends, so the column writes of every step are contiguous.
@@ -130,24 +130,61 @@ struct Work62 {
         }
         sync();
     }
+    // Forward DIF, natural order in, bit-reversed out: the radix-2 levels h = L/2, L/4, .., 1 (pairs
+    // (i, i + h) in blocks of 2h, twiddle tw[h + j] on the difference), taken two at a time as
+    // radix-4 passes over groups {i, i + h/2, i + h, i + 3h/2} (one LDS round trip and one barrier
+    // per two levels), and a last radix-2 pass when log2 L is odd.
     __device__ void fwd() {
-        for (int h = L / 2; h >= 1; h >>= 1) {
+        int h = L / 2;
+        for (; h >= 2; h >>= 2) {
+            const int q = h >> 1;
+            for (int k = t; k < L / 4; k += T) {
+                const int j = k & (q - 1), i = ((k - j) << 2) + j;
+                const u64 x0 = buf[i], x1 = buf[i + q], x2 = buf[i + h], x3 = buf[i + h + q];
+                const u64 y0 = madd(x0, x2, m), y2 = mmul(msub(x0, x2, m), a.tw[h + j], m);
+                const u64 y1 = madd(x1, x3, m), y3 = mmul(msub(x1, x3, m), a.tw[h + j + q], m);
+                const u64 w = a.tw[q + j];
+                buf[i] = madd(y0, y1, m);
+                buf[i + q] = mmul(msub(y0, y1, m), w, m);
+                buf[i + h] = madd(y2, y3, m);
+                buf[i + h + q] = mmul(msub(y2, y3, m), w, m);
+            }
+            sync();
+        }
+        if (h == 1) {
             for (int k = t; k < L / 2; k += T) {
-                const int j = k & (h - 1), i = ((k - j) << 1) + j;
-                const u64 x = buf[i], y = buf[i + h];
-                buf[i] = madd(x, y, m);
-                buf[i + h] = mmul(msub(x, y, m), a.tw[h + j], m);
+                const u64 x = buf[2 * k], y = buf[2 * k + 1];
+                buf[2 * k] = madd(x, y, m);
+                buf[2 * k + 1] = msub(x, y, m);  // tw[1] = 1
             }
             sync();
         }
     }
+    // Inverse DIT, bit-reversed in, natural out: the levels h = 1, 2, .., L/2 (pairs (i, i + h),
+    // the odd entry times twi[h + j]), a first radix-2 pass when log2 L is odd, then two levels
+    // (h, 2h) per radix-4 pass over groups {i, i + h, i + 2h, i + 3h}.
     __device__ void inv() {
-        for (int h = 1; h < L; h <<= 1) {
+        int h = 1;
+        if (LG & 1) {
             for (int k = t; k < L / 2; k += T) {
-                const int j = k & (h - 1), i = ((k - j) << 1) + j;
-                const u64 x = buf[i], y = mmul(buf[i + h], a.twi[h + j], m);
-                buf[i] = madd(x, y, m);
-                buf[i + h] = msub(x, y, m);
+                const u64 x = buf[2 * k], y = buf[2 * k + 1];
+                buf[2 * k] = madd(x, y, m);
+                buf[2 * k + 1] = msub(x, y, m);  // twi[1] = 1
+            }
+            sync();
+            h = 2;
+        }
+        for (; h < L; h <<= 2) {
+            for (int k = t; k < L / 4; k += T) {
+                const int j = k & (h - 1), i = ((k - j) << 2) + j;
+                const u64 wh = a.twi[h + j];
+                const u64 x0 = buf[i], x1 = mmul(buf[i + h], wh, m), x2 = buf[i + 2 * h], x3 = mmul(buf[i + 3 * h], wh, m);
+                const u64 y0 = madd(x0, x1, m), y1 = msub(x0, x1, m), y2 = madd(x2, x3, m), y3 = msub(x2, x3, m);
+                const u64 t2 = mmul(y2, a.twi[2 * h + j], m), t3 = mmul(y3, a.twi[2 * h + j + h], m);
+                buf[i] = madd(y0, t2, m);
+                buf[i + 2 * h] = msub(y0, t2, m);
+                buf[i + h] = madd(y1, t3, m);
+                buf[i + 3 * h] = msub(y1, t3, m);
             }
             sync();
         }
