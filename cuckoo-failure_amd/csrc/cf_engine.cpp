@@ -15,6 +15,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/cuckoo_amd.h"
@@ -94,8 +95,9 @@ int plan_task(cf_engine* e, uint64_t p, int d, int model, size_t nq, Q query, Ta
         return fail(e, CF_E_UNSUPPORTED, "n = " + std::to_string(tp.nm) + " needs a transform longer than 2^" +
                                              std::to_string(kWideMaxLog2) + ", which this build does not run");
     tp.bytes = task_bytes(plan_for(e, tp.nm), tp.F1, (int)tp.queries.size(), tp.lg);
-    // two convolutions of length L per row, plus the tile traffic
+    // two convolutions of length L per row, the tile traffic, and each query's row sums
     tp.cost = 2.0 * tp.nm * ((double)(1 << tp.lg) * (tp.lg + 1)) + 8.0 * tp.nm * (double)tp.nm;
+    for (const QueryDesc& q : tp.queries) tp.cost += 1.5 * (double)q.n * (double)q.n;
     return CF_OK;
 }
 
@@ -696,19 +698,44 @@ int cf_run_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t
         if (rng[i].first < rng[i - 1].second) return fail(e, CF_E_ARGS, "instances share prime/result entries");
     for (const auto& r : rng) std::fill(res + r.first, res + r.second, 0);
     HIP_TRY(e, hipSetDevice(e->device), "hipSetDevice");
+    // the tables: one per (d, model, prime) with every query on it when sharing, else one per
+    // (instance, prime). A shared table has the rows of its largest n; a smaller query reads the
+    // first rows of it, which are those of its own table (row v of V and S depends only on rows
+    // and columns below v, and the cyclic convolutions of length L >= nm never wrap into them).
+    std::map<std::tuple<int, int, uint64_t>, std::vector<QueryDesc>> shared;
     std::vector<TaskPlan> tasks;
     for (size_t i = 0; i < k; ++i) {
         const cf_instance& x = inst[i];
         for (int64_t b = x.prime_off; b < x.prime_off + x.prime_cnt; ++b) {
+            if (e->batch_share) {
+                shared[std::make_tuple(x.d, x.model, primes[b])].push_back(QueryDesc{x.n, x.M, b});
+                continue;
+            }
             TaskPlan tp;
             int rc = plan_task(e, primes[b], x.d, x.model, 1, [&](size_t) { return QueryDesc{x.n, x.M, b}; }, tp);
             if (rc) return rc;
             tasks.push_back(std::move(tp));
         }
     }
+    for (const auto& kv : shared) {
+        const std::vector<QueryDesc>& qs = kv.second;
+        TaskPlan tp;
+        // (a shared table has the transform length of its largest query, so it is refused only
+        // where that query's own table would be)
+        int rc = plan_task(e, std::get<2>(kv.first), std::get<0>(kv.first), std::get<1>(kv.first), qs.size(),
+                           [&](size_t j) { return qs[j]; }, tp);
+        if (rc) return rc;
+        tasks.push_back(std::move(tp));
+    }
     const int rc = execute(e, tasks, (size_t)nres, res, true);
     e->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return rc;
+}
+
+int cf_set_batch_sharing(cf_engine* e, int on) {
+    if (!e) return CF_E_ARGS;
+    e->batch_share = on != 0;
+    return CF_OK;
 }
 
 int cf_set_batch_variant(cf_engine* e, int variant) {

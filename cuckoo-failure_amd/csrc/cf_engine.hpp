@@ -25,6 +25,7 @@ struct cf_engine {
     std::map<int, cf::SlotPlan> plans;  // by nm
     int variant[32] = {};               // task-kernel variant per log2 transform length (cf_autotune)
     int queue_variant = 0;              // work-queue kernel variant (cf_autotune_batch)
+    bool batch_share = true;            // cf_run_batch: one table per (d, model, prime)
     // graph mode (cf_set_graphs): a call's uploads, launches and downloads are one hipGraph,
     // captured once per launch-plan signature and replayed while the signature repeats
     bool graphs = false;

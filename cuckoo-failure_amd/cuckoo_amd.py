@@ -87,6 +87,7 @@ def lib():
         L.cf_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tune)]
         L.cf_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.cf_set_batch_variant.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.cf_set_batch_sharing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.cf_autotune_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Instance), ctypes.c_size_t, ctypes.POINTER(u64),
                                         ctypes.c_size_t, ctypes.POINTER(BatchTune)]
         L.cf_max_batch62.restype = ctypes.c_long
@@ -278,6 +279,13 @@ class Engine:
         rc = lib().cf_set_batch_variant(self._h, variant)
         if rc != CF_OK:
             raise CfError(rc, "bad variant")
+
+    def set_batch_sharing(self, on):
+        """run_batch with one table per (d, model, prime) serving every instance on it (True, the
+        default) or one table per (instance, prime) (False); the residues are the same."""
+        rc = lib().cf_set_batch_sharing(self._h, 1 if on else 0)
+        if rc != CF_OK:
+            raise CfError(rc, "bad engine")
 
     def run_separate(self, n, m, primes, algo="suffix"):
         """N_bad(n, m) mod p for separate tables of lengths m by the device suffix table (algo

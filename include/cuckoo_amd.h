@@ -93,13 +93,19 @@ typedef struct {
     int64_t prime_off, prime_cnt;
 } cf_instance;
 
-/* Uneven batch: every (instance, prime) table of k instances of different n, M and d in one
- * persistent work-queue launch (transforms up to 2^12, i.e. n <= 4096; longer ones in launches
- * of their own), longest first. The reference runs such a batch as one cuckoo_fast process per
- * instance (fast/cuckoo_fast.cu:653-898 with D, QUERIES = "n M", FIRST = 0, COUNT = need); the
- * residues are the same, res[i] = N_bad mod primes[i]. Instances must not share prime/result
- * indices; indices no instance covers are left untouched. */
+/* Uneven batch: k instances of different n, M and d in one persistent work-queue launch
+ * (transforms up to 2^12, i.e. tables of up to 4096 rows; longer ones in launches of their own),
+ * longest table first. Instances with the same d and model that name the same prime share that
+ * prime's table: one table of max n rows answers all their queries, as one cuckoo_fast run
+ * answers every line of its query file from one table per prime (fast/cuckoo_fast.cu:787-876;
+ * rows below n do not depend on the rows beyond). The reference runs such a batch as one
+ * cuckoo_fast process per instance (fast/cuckoo_fast.cu:653-898 with D, QUERIES = "n M",
+ * FIRST = 0, COUNT = need); the residues are the same, res[i] = N_bad mod primes[i]. Instances
+ * must not share prime/result indices; indices no instance covers are left untouched. */
 int cf_run_batch(cf_engine* e, const cf_instance* inst, size_t k, const uint64_t* primes, uint64_t* res);
+/* Table sharing in cf_run_batch: 1 (the default) as above, 0 one table per (instance, prime).
+ * The residues do not depend on it. */
+int cf_set_batch_sharing(cf_engine* e, int on);
 
 /* The reference's direct one-table algorithms on the device, one workgroup per prime, as
  * cross-checks of the suffix-table path: res[b] = N_bad(n, M) mod primes[b] for model 0 (d-subsets)

@@ -35,24 +35,41 @@ def test_uneven_small_instances_vs_reference(eng):
     print(f"uneven batch: {len(insts)} instances, {sum(len(i[4]) for i in insts)} tables", flush=True)
     res = eng.run_batch(insts)
     st = eng.stats()
-    assert st.launches == 1 and st.tasks == sum(x["need"] for x in gold)
+    # instances of one d share each prime's table
+    assert st.launches == 1 and st.tasks == len({(x["d"], p) for x in gold for p in cf.prime_range(0, x["need"])})
     for x, inst, r in zip(gold, insts, res):
         for line in residue_lines("\n".join(x["head"])):
             i, p, n, M, want = line
             assert inst[4][i] == p and r[i] == want, (x["index"], i)
         assert _nbad_hash(inst[4], r) == (x["nbad_bits"], x["nbad_sha256"]), x["index"]
+    # one table per (instance, prime): the same residues
+    eng.set_batch_sharing(False)
+    try:
+        assert eng.run_batch(insts) == res
+        assert eng.stats().tasks == sum(x["need"] for x in gold)
+    finally:
+        eng.set_batch_sharing(True)
 
 
-def test_batch_equals_per_instance_calls(eng):
+@pytest.mark.parametrize("share", [True, False])
+def test_batch_equals_per_instance_calls(eng, share):
     # every queue length class (L = 2^10, 2^11, 2^12) plus one longer table (2^13) that runs in
-    # a launch of its own, models 0 and 1, d from 1 to 5, shared primes across instances
+    # a launch of its own, models 0 and 1, d from 1 to 5, shared primes across instances (with
+    # sharing, primes 0 and 1 of d = 3 are each one table of 5000 rows, a 2^13 transform, that also
+    # carries the queries of n = 4000 and 1025)
     P = cf.prime_range(0, 6)
     spec = [(3, 0, 4000, 9000, P[0:3]), (2, 0, 1500, 2000, P[0:4]), (4, 1, 700, 900, P[1:3]),
             (3, 0, 5000, 12000, P[0:2]), (5, 0, 9, 10, P[0:5]), (1, 0, 100, 150, P[2:6]),
-            (2, 1, 2048, 2049, P[3:5]), (3, 0, 1025, 3000, P[4:6]), (4, 0, 64, 65, [998244353, 1004535809])]
-    res = eng.run_batch(spec)
-    st = eng.stats()
+            (2, 1, 2048, 2049, P[3:5]), (3, 0, 1025, 3000, P[0:2] + P[4:6]), (4, 0, 64, 65, [998244353, 1004535809])]
+    eng.set_batch_sharing(share)
+    try:
+        res = eng.run_batch(spec)
+        st = eng.stats()
+    finally:
+        eng.set_batch_sharing(True)
     assert st.launches == 2 and st.log2_length == 13
+    tables = sum(len(x[4]) for x in spec)
+    assert st.tasks == (tables - 4 if share else tables)  # (d = 3, P0) and (d = 3, P1): three instances, one table
     for (d, model, n, M, primes), r in zip(spec, res):
         want = [row[0] for row in eng.run([(n, M)], primes, d, model=model)]
         assert r == want, (d, model, n, M)
@@ -97,7 +114,7 @@ def test_batch_edges(eng):
         eng.run_batch([(3, 0, 20, 60, [59])])  # prime not above M
     assert ei.value.code == cf.CF_E_ARGS
     with pytest.raises(cf.CfError) as ei:
-        eng.run_batch([(3, 0, 70000, 140000, P[:1])])  # transform beyond 2^16
+        eng.run_batch([(3, 0, 140000, 280000, P[:1])])  # transform beyond 2^17
     assert ei.value.code == cf.CF_E_UNSUPPORTED
 
 
