@@ -332,42 +332,51 @@ struct Task {
         const int n = c.d->NB - 2 - X, NSEG = c.d->NSEG;
         for (int j = t; j < n; j += T) sh.dseg[j] = (unsigned short)c.slots[(X + 2 + j) * NSEG + (X >> 4)];
     }
-    // Slice k (of 64) of block X's deferred entries (both tables): entry gi of the slice range is
-    // table gi / half, tile j = idx >> 8, column (idx >> 4) & 15, row 16 (X+2+j) + (idx & 15).
-    struct DeferSlice {
-        int lo, hi, half;
-    };
-    static CF_HD DeferSlice defer_slice(int X, int k, const Ctx& c) {
-        const int half = (c.d->NB - 2 - X) * 256, E = half > 0 ? 2 * half : 0, chunk = (E + 63) / 64;
-        DeferSlice d;
-        d.lo = k * chunk < E ? k * chunk : E;
-        d.hi = d.lo + chunk < E ? d.lo + chunk : E;
-        d.half = half;
-        return d;
+    // Slice k (of 64) of block X's deferred tiles. The tiles are numbered q = 0 .. 2 (NB-2-X) - 1:
+    // table q & 1 (0: P2, 1: S2), row block X+2 + q/2. Slice k holds the TPS = ceil(tiles / 64)
+    // tiles from k TPS on; virtual thread t moves entry (row t & 15, column (t >> 4) & 15) of its
+    // tiles q = k TPS + t/256 + u T/256 (u < 8, while inside the slice). T/256 is even, so a thread
+    // stays in one table and its tiles lie T/512 row blocks apart: its staging loads differ by
+    // constant offsets, and each store needs one segment lookup (dseg) for its tile.
+    static constexpr int kDeferStep = T / 256;  // tiles per pass of the T virtual threads
+    static_assert(!kDefer || (kDeferStep % 2 == 0 && 8 * kDeferStep * 64 >= 2 * (P::NBMAX - 2)),
+                  "eight passes of T/256 tiles cover a slice, one table per thread");
+    // tiles of slice k (0 if none), and this thread's first tile
+    static CF_HD int defer_count(int t, int X, int k, const Ctx& c, int& q0) {
+        const int ntile = 2 * (c.d->NB - 2 - X);
+        if (ntile <= 0) return 0;
+        const int tps = (ntile + 63) >> 6, lo = k * tps;
+        q0 = lo + (t >> 8);
+        return ntile - lo < tps ? ntile - lo : tps;
     }
+    // Loads of every pass, from the P2 staging base (the S2 staging follows it at 32 NS doubles):
+    // a pass beyond the slice reads staging entries of later rows, or past the S2 staging's end at
+    // most 16 (8 T/512) doubles into the factorial tables that follow it (4 F1 + L >= 8192 more
+    // doubles of the same arena); those values are never stored.
     static CF_HD void defer_load(Thread& th, int t, int X, int k, const Ctx& c) {
-        const DeferSlice d = defer_slice(X, k, c);
-        if (d.lo >= d.hi) return;
-        const uint32_t NS = (uint32_t)c.d->NS, cb = (uint32_t)(X & 1) << 4;
+        int q0 = 0;
+        const int cnt = defer_count(t, X, k, c, q0);
+        if (cnt <= 0) return;
+        const uint32_t NS = (uint32_t)c.d->NS, cb = (uint32_t)(X & 1) << 4, col = (t >> 4) & 15, r = t & 15;
+        // staging entry (row 16 (X+2+j) + r, column cb + col) of table q0 & 1, j = q0/2 + u T/512
+        const uint32_t off0 = (((q0 & 1) ? 32 * NS : 0u) + (cb + col) * NS +
+                               16 * ((uint32_t)X + 2 + (uint32_t)(q0 >> 1)) + r) << 3;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int gi = d.lo + u * T + t, g = gi < d.hi ? gi : d.lo;
-            const int tbl = g >= d.half, idx = g - (tbl ? d.half : 0);
-            const uint32_t j = (uint32_t)idx >> 8, col = ((uint32_t)idx >> 4) & 15, r = (uint32_t)idx & 15;
-            th.pa[u] = ldo(tbl ? c.a.Sst : c.a.Vst, ((cb + col) * NS + 16 * ((uint32_t)X + 2 + j) + r) << 3);
-        }
+        for (int u = 0; u < 8; ++u) th.pa[u] = ldo(c.a.Vst, off0 + (uint32_t)(u * (kDeferStep / 2) * 16 * 8));
     }
     static CF_HD void defer_store(const Thread& th, const Sh& sh, int t, int X, int k, const Ctx& c) {
-        const DeferSlice d = defer_slice(X, k, c);
-        if (d.lo >= d.hi) return;
+        int q0 = 0;
+        const int cnt = defer_count(t, X, k, c, q0), tq = t >> 8;
+        if (cnt <= 0) return;
+        // from the P2 pool base (the S2 pool follows it at pool tiles of 2048 bytes): tile
+        // dseg[j] 16 + X % 16, entry r 16 + col inside it
+        const Off base = ((q0 & 1) ? (Off)c.d->pool << 11 : (Off)0) +
+                         ((Off)(((X & 15) << 8) + ((t & 15) << 4) + ((t >> 4) & 15)) << 3);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int gi = d.lo + u * T + t;
-            if (gi < d.hi) {
-                const int tbl = gi >= d.half, idx = gi - (tbl ? d.half : 0);
-                const int j = idx >> 8, col = (idx >> 4) & 15, r = idx & 15;
-                const int64_t tile = (int64_t)sh.dseg[j] * 16 + (X & 15);
-                (tbl ? c.a.Spool : c.a.Vpool)[tile * 256 + r * 16 + col] = th.pa[u];
+            if (tq + kDeferStep * u < cnt) {
+                const int j = (q0 >> 1) + u * (kDeferStep / 2);
+                *(double*)((char*)c.a.Vpool + (base + ((Off)sh.dseg[j] << 15))) = th.pa[u];
             }
         }
     }
