@@ -146,9 +146,13 @@ struct WideRunner : DevRunner<1, T> {
                 __builtin_amdgcn_s_sleep(CF_WIDE_POLL_SLEEP);
             }
             // the other workgroups' writes, not stale cache lines: within one XCD only this CU's
-            // L1 can be stale (the L2 is shared); otherwise the device-scope fence
+            // L1 can be stale (the L2 is shared); otherwise the device-scope fence. Its invalidate
+            // must be complete before the workgroup barrier below releases the other waves: their
+            // first loads (often of lines this CU read a step earlier) may not hit lines it has
+            // not yet reached.
             if (local) asm volatile("buffer_inv sc0" ::: "memory");
             else __threadfence();
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             *flag = bad;
         }
         __syncthreads();

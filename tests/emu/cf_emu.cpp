@@ -7,7 +7,7 @@
 // exactly with 128-bit integers: its inputs and result are integers, the result is congruent to
 // the product modulo p, and |result| stays below 2^52 (one spare bit under the mantissa).
 //
-//   cf_emu [--lg L] [--defer D] [--reverse] d model p n1 M1 [n2 M2 ...]
+//   cf_emu [--lg L] [--defer D] [--reverse] [--serial-wg 0|1|2] [--poison] d model p n1 M1 [n2 M2 ...]
 //
 // prints one line per query, "n M residue", in the order given, then a line
 // "products N max_ratio R" (the largest |product result| / p seen) and exits 0; a failed check
@@ -41,6 +41,7 @@ std::atomic<bool> g_bad{false};
 std::mutex g_ratio_mu;
 double g_max_ratio = 0;
 bool g_reverse = false;
+bool g_poison = false;
 
 }  // namespace
 
@@ -75,7 +76,18 @@ namespace {
 template <int T>
 struct EmuRunner {
     std::vector<Thread> th = std::vector<Thread>(T);
-    double wacc[T / 64][8];
+    // per destination array: the running sums of each wave (a phase may reduce into several)
+    struct Acc {
+        double* red;
+        double w[T / 64][8];
+    };
+    std::vector<Acc> accs;
+    double (&acc_of(double* red))[T / 64][8] {
+        for (auto& a : accs)
+            if (a.red == red) return a.w;
+        accs.push_back(Acc{red, {}});
+        return accs.back().w;
+    }
     template <class F>
     void phase(F f) {
         if (g_reverse)
@@ -91,6 +103,7 @@ struct EmuRunner {
         double acc[8];
         const int cnt = f(th_, t_, acc);
         const int w = t_ >> 6;
+        double(&wacc)[T / 64][8] = acc_of(red);
         const int lane = t_ & 63, first = g_reverse ? 63 : 0, last = g_reverse ? 0 : 63;
         for (int j = 0; j < 8; ++j) wacc[w][j] = lane == first ? acc[j] : (j < cnt ? reduce(wacc[w][j] + acc[j], m) : wacc[w][j]);
         if (lane == last)
@@ -122,14 +135,47 @@ struct HostBarrier {
     }
 };
 
+// Workgroups run one at a time in a fixed order between group barriers (--serial-wg 1: 0, 1, ..,
+// 15; 2: 15, .., 0): a phase that reads another workgroup's writes without a group barrier between
+// them then sees the same (old or new) value on every run, and the two orders see different ones.
+struct TurnBarrier {
+    std::mutex mu;
+    std::condition_variable cv;
+    int n, arrived = 0, turn;
+    bool rev;
+    TurnBarrier(int k, bool reverse) : n(k), turn(reverse ? k - 1 : 0), rev(reverse) {}
+    int first() const { return rev ? n - 1 : 0; }
+    int after(int g) const { return rev ? g - 1 : g + 1; }
+    void begin(int g) {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return turn == g; });
+    }
+    void wait(int g) {
+        std::unique_lock<std::mutex> lk(mu);
+        turn = ++arrived == n ? (arrived = 0, first()) : after(g);
+        cv.notify_all();
+        cv.wait(lk, [&] { return turn == g; });
+    }
+    void end(int g) {
+        std::lock_guard<std::mutex> lk(mu);
+        turn = after(g);
+        cv.notify_all();
+    }
+};
+
 template <int T>
 struct WideEmuRunner : EmuRunner<T> {
     HostBarrier* bar = nullptr;
+    TurnBarrier* turns = nullptr;
+    int g = 0;
     bool gsync() {
-        bar->wait();
+        if (turns) turns->wait(g);
+        else bar->wait();
         return false;
     }
 };
+
+int g_serial = 0;
 
 struct Job {
     int d = 3, model = 0, lg = 0, defer = -1;
@@ -163,7 +209,10 @@ void describe(Job& j) {
     t.dd = j.d;
     t.model = j.model;
     t.keep = nullptr;
-    j.mem.assign(task_bytes(j.sp, j.F1, (int)j.q.size(), j.lg) / 8 + 64, 0.0);
+    // --poison: the arena starts as NaN, LDS and the threads' register arrays as all-ones bytes
+    // (none is cleared between launches), so a value read before the kernel wrote it fails the
+    // product checks
+    j.mem.assign(task_bytes(j.sp, j.F1, (int)j.q.size(), j.lg) / 8 + 64, g_poison ? std::nan("") : 0.0);
     j.res.assign(j.q.size(), 0);
 }
 
@@ -171,7 +220,9 @@ template <int LG, int DF>
 void run_one_workgroup(Job& j) {
     using TK = Task<LG, LG, DF>;
     auto run = std::make_unique<EmuRunner<Plan<LG>::T>>();
+    if (g_poison) std::memset((void*)run->th.data(), 0xff, sizeof(Thread) * run->th.size());
     auto sh = std::make_unique<typename TK::Sh>();
+    if (g_poison) std::memset((void*)sh.get(), 0xff, sizeof(typename TK::Sh));
     Ctx c;
     c.d = &j.td;
     c.m.p = j.td.p;
@@ -193,17 +244,26 @@ template <int INNER>
 void run_wide(Job& j) {
     using W = Wide<INNER>;
     HostBarrier bar(kWideG);
+    TurnBarrier turns(kWideG, g_serial == 2);
     std::vector<std::thread> ws;
     for (int g = 0; g < kWideG; ++g)
         ws.emplace_back([&, g] {
             auto run = std::make_unique<WideEmuRunner<W::T>>();
+            if (g_poison) std::memset((void*)run->th.data(), 0xff, sizeof(Thread) * run->th.size());  // nor are registers
             run->bar = &bar;
+            run->g = g;
+            if (g_serial) {
+                run->turns = &turns;
+                turns.begin(g);
+            }
             auto sh = std::make_unique<typename W::Sh>();
+            if (g_poison) std::memset((void*)sh.get(), 0xff, sizeof(typename W::Sh));  // LDS is not cleared either
             const Ctx c = W::ctx(&j.td, j.q.data(), j.sp.segs.data(), j.mem.data(), g, j.model, j.d);
             const WideArena w = wide_arena_of<INNER>(j.mem.data(), j.td);
             W::run_task(*run, *sh, c, w, g);
             if (g == 0)
                 for (size_t k = 0; k < j.q.size(); ++k) j.res[(size_t)j.q[k].out] = (uint64_t)W::TK::result((int)k, c);
+            if (g_serial) turns.end(g);
         });
     for (auto& t : ws) t.join();
 }
@@ -229,13 +289,15 @@ int main(int argc, char** argv) {
         if (!std::strcmp(argv[i], "--lg") && i + 1 < argc) j.lg = std::atoi(argv[++i]);
         else if (!std::strcmp(argv[i], "--defer") && i + 1 < argc) j.defer = std::atoi(argv[++i]);
         else if (!std::strcmp(argv[i], "--reverse")) g_reverse = true;
+        else if (!std::strcmp(argv[i], "--serial-wg") && i + 1 < argc) g_serial = std::atoi(argv[++i]);
+        else if (!std::strcmp(argv[i], "--poison")) g_poison = true;
         else {
             std::fprintf(stderr, "unknown option %s\n", argv[i]);
             return 2;
         }
     }
     if (argc - i < 5 || (argc - i - 3) % 2) {
-        std::fprintf(stderr, "usage: cf_emu [--lg L] [--defer D] [--reverse] d model p n1 M1 [n2 M2 ...]\n");
+        std::fprintf(stderr, "usage: cf_emu [--lg L] [--defer D] [--reverse] [--serial-wg 0|1|2] [--poison] d model p n1 M1 [n2 M2 ...]\n");
         return 2;
     }
     j.d = std::atoi(argv[i]);

@@ -2,7 +2,10 @@
 compiled for the host, every modular product checked exactly with 128-bit integers), against the
 oracle: each transform-length class of the one-workgroup kernel (both tile-flush variants), the
 16-workgroup kernel of tables beyond 16384 rows with more than 8 queries, both models, and every
-phase's threads run in reverse order (a phase must not depend on its own writes)."""
+phase's threads run in reverse order (a phase must not depend on its own writes). The 16-workgroup
+kernels also run with their workgroups one at a time between group barriers, in ascending and in
+descending order: a read of another workgroup's data that no group barrier orders after its write
+then gives different answers in the two orders."""
 import os
 import subprocess
 
@@ -22,7 +25,7 @@ def emu():
     return EMU
 
 
-def run_emu(emu, d, model, p, queries, lg=None, defer=None, reverse=False):
+def run_emu(emu, d, model, p, queries, lg=None, defer=None, reverse=False, serial=0, poison=False):
     argv = [emu]
     if lg:
         argv += ["--lg", str(lg)]
@@ -30,6 +33,10 @@ def run_emu(emu, d, model, p, queries, lg=None, defer=None, reverse=False):
         argv += ["--defer", str(defer)]
     if reverse:
         argv += ["--reverse"]
+    if serial:
+        argv += ["--serial-wg", str(serial)]
+    if poison:
+        argv += ["--poison"]
     argv += [str(d), str(model), str(p)] + [str(x) for q in queries for x in q]
     r = subprocess.run(argv, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -83,6 +90,27 @@ def test_sixteen_workgroup_kernel_with_many_queries(emu):
     assert got == want
     got, _ = run_emu(emu, 3, 0, P0, qs[:9], lg=16, reverse=True)
     assert got == want[:9]
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("serial", [1, 2])
+def test_sixteen_workgroup_kernel_in_fixed_workgroup_orders(emu, serial):
+    # the hand-overs across workgroups 0 -> 1 (row 256), the per-workgroup query slots (more than
+    # 8 queries, and row nm right after the last step), from a poisoned start: arena NaN, LDS and
+    # the threads' register arrays all-ones (none of them is cleared between launches)
+    qs = [(300, 900), (300, 301), (299, 500), (250, 700), (200, 400), (180, 181), (150, 600), (120, 240),
+          (100, 800), (90, 91), (64, 65), (50, 70), (40, 400), (30, 31), (20, 40), (10, 11), (9, 10), (8, 200),
+          (300, 450), (5, 6)]
+    got, _ = run_emu(emu, 3, 0, P0, qs, lg=16, serial=serial, poison=True)
+    assert got == oracle.nbad(3, P0, qs)
+
+
+def test_poisoned_start_changes_nothing(emu):
+    # one-workgroup kernels of each flush variant from a poisoned arena, LDS and registers
+    qs = [(300, 900), (300, 301), (64, 65)]
+    for lg, defer in ((12, None), (14, None), (14, 0)):
+        got, _ = run_emu(emu, 3, 0, P1, qs, lg=lg, defer=defer, poison=True)
+        assert got == oracle.nbad(3, P1, qs), (lg, defer)
 
 
 @pytest.mark.slow
