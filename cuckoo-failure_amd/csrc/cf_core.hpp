@@ -234,11 +234,11 @@ CF_HD int64_t arena_doubles(int pool, int NS, int F1, int q_cnt, int L) {
 }
 
 // A table of more than 16384 rows (16 workgroups per prime, transforms of L = 16 N2 = 2^16 or
-// 2^17) holds after that arena its exchange rows (16 N2 doubles), outer twiddles (16 x 8 N2/16),
-// the per-workgroup query sums (16 x the query slots, at least 8) and the hand-over entry (16).
+// 2^17) holds after that arena its exchange rows (16 N2 doubles), the per-workgroup query sums
+// (16 x the query slots, at least 8) and the hand-over entry (16).
 CF_HD int wide_qslots(int q_cnt) { return q_cnt > 8 ? q_cnt : 8; }
 CF_HD int64_t wide_extra_doubles(int q_cnt, int log2L) {
-    return 24 * (int64_t(1) << (log2L - 4)) + 16 * (int64_t)wide_qslots(q_cnt) + 16;
+    return 16 * (int64_t(1) << (log2L - 4)) + 16 * (int64_t)wide_qslots(q_cnt) + 16;
 }
 
 CF_HD Arena arena_of(double* base, const TaskDesc& t) {

@@ -32,7 +32,6 @@ static_assert(kWideG == 16, "wide_extra_doubles() assumes 16 workgroups per prim
 // Extra arena of a wide task, after the one-workgroup layout of length L = 16 N2 (doubles).
 struct WideArena {
     double* exch;   // 16 x N2 exchange rows
-    double* twout;  // per workgroup 8 T: w^(n2 2^k) (k < 4) then w^-(n2 2^k), at k T + t
     double* qpart;  // [g][qs] per-workgroup query sums
     double* hand;   // [0]: the hand-over entry when it crosses workgroups
     int qs;         // query slots per workgroup: max(8, the task's query count)
@@ -42,8 +41,7 @@ CF_HD WideArena wide_arena_of(double* base, const TaskDesc& t) {
     constexpr int64_t N2 = int64_t(1) << INNER;
     WideArena w;
     w.exch = base + t.arena + arena_doubles(t.pool, t.NS, t.F1, t.q_cnt, (int)(16 * N2));
-    w.twout = w.exch + 16 * N2;
-    w.qpart = w.twout + 8 * N2;
+    w.qpart = w.exch + 16 * N2;
     w.qs = wide_qslots(t.q_cnt);
     w.hand = w.qpart + (int64_t)kWideG * w.qs;
     return w;
@@ -53,10 +51,14 @@ template <int INNER>
 struct Wide {
     static constexpr int LOG2L = INNER + 4;
     using TK = Task<INNER, LOG2L>;  // stage functions (wide element map), LDS layout
-    using Sh = Shared<INNER, LOG2L>;
     using E = Engine<INNER>;        // the inner transform
     using P = Plan<INNER>;
     static constexpr int T = P::T, G = kWideG, NW = T / 64, N2 = 1 << INNER;
+    // LDS: the one-workgroup layout, and the outer twiddles of this workgroup's columns:
+    // otw[k T + t] = w^(n2 2^k) (k < 4), otw[(4 + k) T + t] = w^-(n2 2^k), n2 = T g + t
+    struct Sh : Shared<INNER, LOG2L> {
+        double otw[8 * T];
+    };
 
     // The context of workgroup g: the wide element map and its slice of the spectrum.
     static CF_HD Ctx ctx(const TaskDesc* d, const QueryDesc* qs, const int* slots, double* mem, int g, int model, int dd) {
@@ -78,35 +80,55 @@ struct Wide {
     }
 
     // ---- the outer step ----
-    static CF_HD void outer_twiddles(const WideArena& w, int g, int t, const TaskDesc& d, const Mod& m) {
-        double* tw = w.twout + 8 * T * g;
+    static CF_HD void outer_twiddles(Sh& sh, int g, int t, const TaskDesc& d, const Mod& m) {
         const uint64_t n2 = (uint64_t)(T * g + t);
         double f = powmod(d.wroot, n2, m), i = powmod(d.wrooti, n2, m);
         for (int k = 0; k < 4; ++k) {
-            tw[k * T + t] = f, tw[4 * T + k * T + t] = i;
+            sh.otw[k * T + t] = f, sh.otw[4 * T + k * T + t] = i;
             f = reduce(mulred(f, f, m), m), i = reduce(mulred(i, i, m), m);
         }
     }
     // this thread's w^(n2 2^k), k < 4 (dir 0), or w^-(n2 2^k) (dir 1)
-    static CF_HD void outer_bases(double (&b4)[4], const WideArena& w, int g, int t, int dir) {
-        const double* src = w.twout + 8 * T * g + 4 * T * dir + t;
+    static CF_HD void outer_bases(double (&b4)[4], const Sh& sh, int t, int dir) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) b4[k] = src[k * T];
+        for (int k = 0; k < 4; ++k) b4[k] = sh.otw[(4 * dir + k) * T + t];
     }
-    static CF_HD void outer_fwd(Thread& th, const WideArena& w, int g, int t, const TaskDesc& d, const Mod& m) {
+    // Power tables of g for step v (lo_pre[i] = v! g^i, lo_post[i] = g^i / v!, hi[j] = g^(64 j)):
+    // thread t makes hi[t + T i] as (g^64)^t times (g^(64 T))^i, a chain of about 20 products
+    // instead of one power per entry.
+    static CF_HD void powtab(Sh& sh, int t, int v, double gv, const Ctx& c) {
+        const Mod& m = c.m;
+        if (t < 128) {
+            const double x = powmod(gv, (uint64_t)(t & 63), m);
+            if (t < 64) sh.lo_pre[t] = reduce(mulred(x, c.a.fact[v], m), m);
+            else sh.lo_post[t - 64] = reduce(mulred(x, c.a.ifact[v], m), m);
+        }
+        const int nhi = (c.d->nm >> 6) + 1;
+        if (t < nhi) {
+            const double g64 = powmod(gv, 64, m), step = powmod(g64, (uint64_t)T, m);
+            double h = powmod(g64, (uint64_t)t, m);
+            for (int j = t; j < nhi; j += T) {
+                sh.hi[j] = h;
+                h = reduce(mulred(h, step, m), m);
+            }
+        }
+    }
+    static CF_HD void outer_fwd(Thread& th, const Sh& sh, int t, const TaskDesc& d, const Mod& m, const WideArena& w,
+                                int g) {
         double b4[4];
-        outer_bases(b4, w, g, t, 0);
+        outer_bases(b4, sh, t, 0);
         reduce16(th.x, m);
         dft_groups<16>(th.x, d.W, m);
         twiddle_chain(th.x, b4, m);
 #pragma unroll
         for (int k = 0; k < 16; ++k) w.exch[k * N2 + T * g + t] = th.x[k];
     }
-    static CF_HD void outer_inv(Thread& th, const WideArena& w, int g, int t, const TaskDesc& d, const Mod& m) {
+    static CF_HD void outer_inv(Thread& th, const Sh& sh, int t, const TaskDesc& d, const Mod& m, const WideArena& w,
+                                int g) {
 #pragma unroll
         for (int k = 0; k < 16; ++k) th.x[k] = w.exch[k * N2 + T * g + t];
         double b4[4];
-        outer_bases(b4, w, g, t, 1);
+        outer_bases(b4, sh, t, 1);
         twiddle_chain(th.x, b4, m);
         reduce16(th.x, m);
         dft_groups<16>(th.x, d.Wi, m);
@@ -124,10 +146,11 @@ struct Wide {
     // The inner part of a convolution (between the two group barriers). The factorials of the
     // stage output (off: 0 stage 1, 1 stage 2) are fetched into th.pa on the way; with qn > 0 the
     // first phase also moves this workgroup's sums of queries 0 .. qn-1 of the phase before the
-    // barrier (in LDS) to its slots, which workgroup 0 reads after the next group barrier.
+    // barrier (in LDS) to its slots, which workgroup 0 reads after the next group barrier; with
+    // tab the second phase makes the power tables of g(v) for the stage-2 phases that follow.
     template <class R>
     static CF_HD void inner(R& run, Sh& sh, const WideArena& w, int g, int v, int off, bool make_hat,
-                            const Ctx& c, int qn = 0) {
+                            const Ctx& c, int qn = 0, bool tab = false, double gv = 0.0) {
         run.phase([&](Thread& th, int t) {
             if (qn) query_slots(sh, w, g, t, 0, qn, c.m);
             inner_load(th, w, g, t);
@@ -138,6 +161,7 @@ struct Wide {
             run.phase([&](Thread& th, int t) {
                 if (!make_hat && r == P::NTW - 1) TK::fetch_hat(th, t, c);
                 E::template fwdr<r>(th, sh, t, *c.d, c.m);
+                if (r == 1 && tab) powtab(sh, t, v, gv, c);
             });
         });
         run.phase([&](Thread& th, int t) { E::final_round(make_hat, th, sh, t, *c.d, c.m, c.a.hat); });
@@ -220,14 +244,14 @@ struct Wide {
         run.phase([&](Thread&, int t) {
             TK::inv_table(t, c);
             E::twiddles(sh, t, d, c.m);
-            outer_twiddles(w, g, t, d, c.m);
+            outer_twiddles(sh, g, t, d, c.m);
             if (g == 0) TK::query_init(t, c);
         });
         if (run.gsync()) return false;
         // kernel spectrum (times 1/L), this workgroup's slice
         run.phase([&](Thread& th, int t) {
             TK::kernel_input(th, t, c);
-            outer_fwd(th, w, g, t, d, c.m);
+            outer_fwd(th, sh, t, d, c.m, w, g);
         });
         if (run.gsync()) return false;
         inner(run, sh, w, g, 0, 0, true, c);
@@ -250,7 +274,7 @@ struct Wide {
                     run.phase([&](Thread& th, int t) {
                         TK::stage1_pre(th, sh, t, v, now, c);
                         query_wave(run, th, sh, t, v, 0, nact, c);
-                        outer_fwd(th, w, g, t, d, c.m);
+                        outer_fwd(th, sh, t, d, c.m, w, g);
                     });
                 } else {
                     run.phase([&](Thread& th, int t) {
@@ -259,21 +283,20 @@ struct Wide {
                     });
                     run.phase([&](Thread&, int t) { query_slots(sh, w, g, t, 0, 8, c.m); });
                     query_groups(run, sh, w, g, v, nact, c);
-                    run.phase([&](Thread& th, int t) { outer_fwd(th, w, g, t, d, c.m); });
+                    run.phase([&](Thread& th, int t) { outer_fwd(th, sh, t, d, c.m, w, g); });
                 }
                 if (run.gsync()) return false;
-                inner(run, sh, w, g, v, 0, false, c, nact <= 8 ? nact : 0);
+                inner(run, sh, w, g, v, 0, false, c, nact <= 8 ? nact : 0, s2, gv);
                 if (run.gsync()) return false;
                 run.phase([&](Thread& th, int t) {
                     query_total(w, g, t, v, nact, c);
-                    outer_inv(th, w, g, t, d, c.m);
+                    outer_inv(th, sh, t, d, c.m, w, g);
                     TK::stage1_post(th, sh, t, v, c);
-                    if (s2) TK::powtab(sh, t, v, gv, c);
                 });
             } else {
                 run.phase([&](Thread&, int t) {
                     TK::zero_col(c.a.Sst, t, v, v, nm, c);
-                    if (s2) TK::powtab(sh, t, v, gv, c);
+                    if (s2) powtab(sh, t, v, gv, c);
                 });
             }
             const bool next = s2 && (v & 15) != 15;
@@ -281,13 +304,13 @@ struct Wide {
                 if (v > c.v0) {
                     run.phase([&](Thread& th, int t) {
                         TK::stage2_pre(th, sh, t, v, c);
-                        outer_fwd(th, w, g, t, d, c.m);
+                        outer_fwd(th, sh, t, d, c.m, w, g);
                     });
                     if (run.gsync()) return false;
                     inner(run, sh, w, g, v, 1, false, c);
                     if (run.gsync()) return false;
                     run.phase([&](Thread& th, int t) {
-                        outer_inv(th, w, g, t, d, c.m);
+                        outer_inv(th, sh, t, d, c.m, w, g);
                         TK::stage2_post(th, sh, t, v, true, c);
                         if (next) TK::fetch_next(th, sh, t, v, c);
                     });
