@@ -349,20 +349,24 @@ struct Task {
         q0 = lo + (t >> 8);
         return ntile - lo < tps ? ntile - lo : tps;
     }
-    // Loads of every pass, from the P2 staging base (the S2 staging follows it at 32 NS doubles):
-    // a pass beyond the slice reads staging entries of later rows, or past the S2 staging's end at
-    // most 16 (8 T/512) doubles into the factorial tables that follow it (4 F1 + L >= 8192 more
-    // doubles of the same arena); those values are never stored.
+    // Loads of every pass, from the P2 staging base (the S2 staging follows it at 32 NS doubles).
+    // A pass beyond the slice loads the first pass's entry again (a cache hit, no HBM traffic); that
+    // entry itself, when this thread has no tile in the slice, is a staging entry of a later row or
+    // lies past the S2 staging's end at most 16 (8 T/512) doubles into the factorial tables that
+    // follow it (4 F1 + L >= 8192 more doubles of the same arena); such values are never stored.
     static CF_HD void defer_load(Thread& th, int t, int X, int k, const Ctx& c) {
         int q0 = 0;
-        const int cnt = defer_count(t, X, k, c, q0);
+        const int cnt = defer_count(t, X, k, c, q0), tq = t >> 8;
         if (cnt <= 0) return;
         const uint32_t NS = (uint32_t)c.d->NS, cb = (uint32_t)(X & 1) << 4, col = (t >> 4) & 15, r = t & 15;
         // staging entry (row 16 (X+2+j) + r, column cb + col) of table q0 & 1, j = q0/2 + u T/512
         const uint32_t off0 = (((q0 & 1) ? 32 * NS : 0u) + (cb + col) * NS +
                                16 * ((uint32_t)X + 2 + (uint32_t)(q0 >> 1)) + r) << 3;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) th.pa[u] = ldo(c.a.Vst, off0 + (uint32_t)(u * (kDeferStep / 2) * 16 * 8));
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t du = tq + kDeferStep * u < cnt ? (uint32_t)(u * (kDeferStep / 2) * 16 * 8) : 0u;
+            th.pa[u] = ldo(c.a.Vst, off0 + du);
+        }
     }
     static CF_HD void defer_store(const Thread& th, const Sh& sh, int t, int X, int k, const Ctx& c) {
         int q0 = 0;
