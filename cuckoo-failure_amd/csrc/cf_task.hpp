@@ -284,25 +284,31 @@ struct Task {
         const int NSEG = c.d->NSEG;
         for (int j = t; j < c.d->NB - 1 - X; j += T) tl[j] = c.slots[(X + 1 + j) * NSEG + (X >> 4)] * 16 + (X & 15);
     }
-    static CF_HD void flush(const Sh& sh, double* pool, const double* st, int t, int X, const Ctx& c) {
+    // Both tables in each trip (their staging and pools share one layout and tile numbering), so
+    // every thread has 32 loads in flight before it stores.
+    static CF_HD void flush(const Sh& sh, int t, int X, const Ctx& c) {
         const int* tl = (const int*)sh.buf;
         const int n = (c.d->NB - 1 - X) * 256;
         const uint32_t NS = (uint32_t)c.d->NS, row0 = 16 * (uint32_t)(X + 1), cb = (uint32_t)(X & 1) << 4;
         const int gt = c.eb + t, GT = c.es;  // this thread among all threads of the task
         for (int base = 0; base < n; base += 16 * GT) {
-            double y[16];
+            double yv[16], ys[16];
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
                 const int idx = base + u * GT + gt, ci = idx < n ? idx : 0;
                 const uint32_t j = (uint32_t)ci >> 8, col = ((uint32_t)ci >> 4) & 15, r = (uint32_t)ci & 15;
-                y[u] = ldo(st, ((cb + col) * NS + row0 + 16 * j + r) << 3);
+                const uint32_t off = ((cb + col) * NS + row0 + 16 * j + r) << 3;
+                yv[u] = ldo(c.a.Vst, off);
+                ys[u] = ldo(c.a.Sst, off);
             }
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
                 const int idx = base + u * GT + gt;
                 if (idx < n) {
                     const int col = (idx >> 4) & 15, r = idx & 15;
-                    pool[(int64_t)tl[idx >> 8] * 256 + (r << 4) + col] = y[u];
+                    const int64_t at = (int64_t)tl[idx >> 8] * 256 + (r << 4) + col;
+                    c.a.Vpool[at] = yv[u];
+                    c.a.Spool[at] = ys[u];
                 }
             }
         }
@@ -600,10 +606,7 @@ struct Task {
                     run.phase([&](Thread&, int t) { flush_head(t, v >> 4, c); });
                 } else {
                     run.phase([&](Thread&, int t) { flush_tiles(sh, t, v >> 4, c); });
-                    run.phase([&](Thread&, int t) {
-                        flush(sh, c.a.Vpool, c.a.Vst, t, v >> 4, c);
-                        flush(sh, c.a.Spool, c.a.Sst, t, v >> 4, c);
-                    });
+                    run.phase([&](Thread&, int t) { flush(sh, t, v >> 4, c); });
                 }
             }
         }

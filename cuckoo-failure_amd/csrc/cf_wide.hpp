@@ -346,10 +346,7 @@ struct Wide {
             if ((v & 15) == 15) {
                 if (!cross && run.gsync()) return false;  // every workgroup's staging columns
                 run.phase([&](Thread&, int t) { TK::flush_tiles(sh, t, v >> 4, c); });
-                run.phase([&](Thread&, int t) {
-                    TK::flush(sh, c.a.Vpool, c.a.Vst, t, v >> 4, c);
-                    TK::flush(sh, c.a.Spool, c.a.Sst, t, v >> 4, c);
-                });
+                run.phase([&](Thread&, int t) { TK::flush(sh, t, v >> 4, c); });
                 if (run.gsync()) return false;  // the tiles, before any row fetch
             }
         }
