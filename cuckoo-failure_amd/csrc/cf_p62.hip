@@ -18,10 +18,12 @@
 // with k[t] = 1/t (1 <= t < nm) and the result N_bad = M_j! tot_j. One workgroup runs one prime:
 // the cyclic convolution of length L = max(16, next_pow2(nm)) <= 2^14 in LDS (DIF forward, natural
 // in and bit-reversed out; pointwise product with the kernel's spectrum, made by the same forward
-// and scaled by 1/L; DIT inverse back to natural order; two radix-2 levels per LDS pass). The tables are packed
-// triangles in HBM (P2 row p at p(p-1)/2, S2 row e at e(e-1)/2 with entry q at q-1); the 16
-// columns of the current column block go to column-major staging first and are copied into the
-// rows when the block ends, so the column writes of every step are contiguous.
+// and scaled by 1/L; DIT inverse back to natural order; radix-16 register passes). The tables are
+// kept as the main engine keeps them (cf_core.hpp tab_off, cf_host.cpp plan_slots): 16 x 16 tiles
+// in a pool whose 32 KB segments are reused once their row block has been read (about nm^2/4
+// entries per table), with the columns of the current column block staged column-major (two
+// blocks, by column-block parity) and copied into their tiles when the block ends, so the column
+// writes of every step are contiguous.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -66,12 +68,22 @@ __device__ __forceinline__ u64 shfl_xor_u64(u64 x, int lane_mask) {
 
 constexpr int kQG = 8;  // queries reduced together
 
+// The transform of length L = 2^LG as register passes: F16 = LG / 4 passes of radix 16 and, when
+// LG % 4 = r > 0, one pass of radix RP = 2^r (the last forward pass, the first inverse pass).
+// A radix-16 pass has G1 = L / 16 groups; thread t holds groups t, t + T, .. (two at L = 2^14: 512
+// threads with 256 registers each, rather than 1024 with 128).
 template <int LG>
 struct P62 {
     static constexpr int L = 1 << LG;
-    static constexpr int T = L / 2 < 64 ? 64 : (L / 2 < kP62Threads ? L / 2 : kP62Threads);  // whole waves
+    static constexpr int G1 = L / 16;
+    static constexpr int VT = LG == 14 ? 2 : 1;
+    static constexpr int T = G1 / VT < 64 ? 64 : G1 / VT;  // whole waves
     static constexpr int NW = T / 64;
     static constexpr int HI = L / 64 + 2;
+    static constexpr int F16 = LG / 4, RP = 1 << (LG % 4), NF = F16 + (LG % 4 ? 1 : 0);
+    static constexpr int RM = LG % 4 ? RP : 16;  // radix of the last forward / first inverse pass
+    static constexpr int NSEG = ((L + 1 + 15) / 16 + 15) / 16;  // segments per row block, nm <= L
+    static_assert(T <= kP62Threads, "workgroup size");
 };
 
 // The arena of one task (64-bit words), in this order.
@@ -79,13 +91,13 @@ struct Arena62 {
     u64 *fact, *ifact, *inv, *ifact2, *mont;  // F1 each (Montgomery form; mont[i] = i R mod p)
     u64 *tw, *twi;                            // L each: tw[h + j] = w_{2h}^j (and inverses)
     u64* hat;                                 // L: spectrum of the kernel times 1/L (bit-reversed order)
-    u64 *Vt, *St;                             // packed triangles
-    u64 *Vs, *Ss;                             // staging: 16 columns x NS rows, column-major
+    u64 *Vpool, *Spool;                       // tiles of 16 x 16 entries, row-major
+    u64 *Vs, *Ss;                             // staging: 2 blocks x 16 columns x NS rows, column-major
     u64* qtot;                                // 2 per query: Horner accumulator, Q = g(M)
 };
 __device__ __forceinline__ Arena62 arena62(u64* mem, const Task62& d) {
     Arena62 a;
-    const int64_t F1 = d.F1, L = d.L, NS = d.nm + 1, tri = (int64_t)d.nm * (d.nm + 1) / 2;
+    const int64_t F1 = d.F1, L = d.L, NS = d.NS, tiles = 256 * d.pool;
     a.fact = mem + d.arena;
     a.ifact = a.fact + F1;
     a.inv = a.ifact + F1;
@@ -94,47 +106,48 @@ __device__ __forceinline__ Arena62 arena62(u64* mem, const Task62& d) {
     a.tw = a.mont + F1;
     a.twi = a.tw + L;
     a.hat = a.twi + L;
-    a.Vt = a.hat + L;
-    a.St = a.Vt + tri;
-    a.Vs = a.St + tri;
-    a.Ss = a.Vs + 16 * NS;
-    a.qtot = a.Ss + 16 * NS;
+    a.Vpool = a.hat + L;
+    a.Spool = a.Vpool + tiles;
+    a.Vs = a.Spool + tiles;
+    a.Ss = a.Vs + 32 * NS;
+    a.qtot = a.Ss + 32 * NS;
     return a;
 }
 
 template <int LG>
 struct Work62 {
     using PP = P62<LG>;
-    static constexpr int L = PP::L, T = PP::T, NW = PP::NW;
+    static constexpr int L = PP::L, T = PP::T, NW = PP::NW, G1 = PP::G1, F16 = PP::F16, NF = PP::NF, RM = PP::RM;
     const Task62& d;
     const Mod62 m;
     const Arena62 a;
     const QueryDesc* q;
     u64* buf;
     u64 *glo, *ghi, *red;
+    u64 *bw, *bwi, *c16, *c16i;  // LDS: w^e and w^-e (e < G1) for w of order L; w_16^r and w_16^-r (r < 8)
+    const int* slots;            // segment slots of the plan, Y * NSEG + s
+    int* segrow;                 // LDS: the segment slots of the current row block
     int t;
 
-    __device__ void sync() const { __syncthreads(); }
-    __device__ u64 g_of(int v) const {
+    __device__ __forceinline__ void sync() const { __syncthreads(); }
+    __device__ __forceinline__ u64 g_of(int v) const {
         if (d.model) return mpow(a.mont[v], (u64)d.dd, m);
         if (v < d.dd) return 0;
         return mmul(mmul(a.fact[v], a.ifact[d.dd], m), a.ifact[v - d.dd], m);
     }
-    __device__ u64 gpow(int e) const { return mmul(ghi[e >> 6], glo[e & 63], m); }
-    // g^i (i < 64) and g^(64 j) (64 j <= nm + 64)
-    __device__ void power_tables(u64 g) {
+    __device__ __forceinline__ u64 gpow(int e) const { return mmul(ghi[e >> 6], glo[e & 63], m); }
+    // g^i (i < 64) and g^(64 j) (64 j <= nm + 64); the caller's next barrier publishes them
+    __device__ __forceinline__ void power_tables(u64 g) {
         const int nhi = (d.nm >> 6) + 2;
         for (int i = t; i < 64 + nhi; i += T) {
             if (i < 64) glo[i] = mpow(g, (u64)i, m);
             else ghi[i - 64] = mpow(mpow(g, 64, m), (u64)(i - 64), m);
         }
-        sync();
     }
-    // Forward DIF, natural order in, bit-reversed out: the radix-2 levels h = L/2, L/4, .., 1 (pairs
-    // (i, i + h) in blocks of 2h, twiddle tw[h + j] on the difference), taken two at a time as
-    // radix-4 passes over groups {i, i + h/2, i + h, i + 3h/2} (one LDS round trip and one barrier
-    // per two levels), and a last radix-2 pass when log2 L is odd.
-    __device__ void fwd() {
+    // Forward DIF, natural order in, bit-reversed out, by radix-2 levels h = L/2, .., 1 (pairs (i, i + h)
+    // in blocks of 2h, twiddle tw[h + j] on the difference), two levels per LDS pass. Used once per
+    // task, for the kernel's spectrum; the v-loop runs the same butterflies as the register passes below.
+    __device__ __forceinline__ void fwd() {
         int h = L / 2;
         for (; h >= 2; h >>= 2) {
             const int q = h >> 1;
@@ -160,58 +173,200 @@ struct Work62 {
             sync();
         }
     }
-    // Inverse DIT, bit-reversed in, natural out: the levels h = 1, 2, .., L/2 (pairs (i, i + h),
-    // the odd entry times twi[h + j]), a first radix-2 pass when log2 L is odd, then two levels
-    // (h, 2h) per radix-4 pass over groups {i, i + h, i + 2h, i + 3h}.
-    __device__ void inv() {
-        int h = 1;
-        if (LG & 1) {
-            for (int k = t; k < L / 2; k += T) {
-                const u64 x = buf[2 * k], y = buf[2 * k + 1];
-                buf[2 * k] = madd(x, y, m);
-                buf[2 * k + 1] = msub(x, y, m);  // twi[1] = 1
-            }
-            sync();
-            h = 2;
-        }
-        for (; h < L; h <<= 2) {
-            for (int k = t; k < L / 4; k += T) {
-                const int j = k & (h - 1), i = ((k - j) << 2) + j;
-                const u64 wh = a.twi[h + j];
-                const u64 x0 = buf[i], x1 = mmul(buf[i + h], wh, m), x2 = buf[i + 2 * h], x3 = mmul(buf[i + 3 * h], wh, m);
-                const u64 y0 = madd(x0, x1, m), y1 = msub(x0, x1, m), y2 = madd(x2, x3, m), y3 = msub(x2, x3, m);
-                const u64 t2 = mmul(y2, a.twi[2 * h + j], m), t3 = mmul(y3, a.twi[2 * h + j + h], m);
-                buf[i] = madd(y0, t2, m);
-                buf[i + 2 * h] = msub(y0, t2, m);
-                buf[i + h] = madd(y1, t3, m);
-                buf[i + 3 * h] = msub(y1, t3, m);
-            }
-            sync();
+
+    // ---- the v-loop's transforms: groups of R entries in registers, one LDS round trip per pass ----
+    // LDS word of transform entry i: bits 0-4 XORed with bits 4-8 (a bijection of [0, L)), so the
+    // strided groups of every pass spread over the banks.
+    __device__ __forceinline__ static int sw(int i) { return i ^ ((i >> 4) & 31); }
+    template <int R>
+    __device__ __forceinline__ void ld(u64 (&x)[16], int i0, int s) const {
+#pragma unroll
+        for (int k = 0; k < R; ++k) x[k] = buf[sw(i0 + k * s)];
+    }
+    template <int R>
+    __device__ __forceinline__ void st(const u64 (&x)[16], int i0, int s) const {
+#pragma unroll
+        for (int k = 0; k < R; ++k) buf[sw(i0 + k * s)] = x[k];
+    }
+    // Radix-R DIF over the levels h, h/2, .., 2h/R of the group x[k] = entry i0 + k (2h/R), i0 = j mod
+    // (2h/R) within its block of 2h, with b = w_2h^j: level l pairs (k, k + e), e = R / 2^(l+1), and
+    // multiplies the difference by w_(2h/2^l)^(j + (k mod e) 2h/R) = b^(2^l) w_16^((16/R) 2^l (k mod e)).
+    // The same butterflies, products and order as the radix-2 levels of fwd().
+    template <int R>
+    __device__ __forceinline__ void dif(u64 (&x)[16], u64 b) const {
+        constexpr int LGR = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : 1;
+#pragma unroll
+        for (int l = 0; l < LGR; ++l) {
+            const int e = R >> (l + 1);
+            u64 tw[8];
+#pragma unroll
+            for (int r = 0; r < e; ++r) tw[r] = r == 0 ? b : mmul(b, c16[((16 / R) << l) * r], m);
+#pragma unroll
+            for (int k0 = 0; k0 < R; k0 += 2 * e)
+#pragma unroll
+                for (int r = 0; r < e; ++r) {
+                    const u64 u = x[k0 + r], w = x[k0 + r + e];
+                    x[k0 + r] = madd(u, w, m);
+                    x[k0 + r + e] = mmul(msub(u, w, m), tw[r], m);
+                }
+            if (l + 1 < LGR) b = mmul(b, b, m);
         }
     }
-    // buf = buf (*) k (cyclic, length L), natural order in and out
-    __device__ void convolve() {
-        fwd();
-        for (int i = t; i < L; i += T) buf[i] = mmul(buf[i], a.hat[i], m);
+    // Radix-R DIT over the levels h, 2h, .., Rh/2 of the group x[k] = entry i0 + k h (j = i0 mod h),
+    // with c = w_(Rh)^-j: level l pairs (k, k + e), e = 2^l, and multiplies the second entry by
+    // w_(2h 2^l)^-(j + (k mod e) h) = c^(R / 2^(l+1)) w_16^-((8 >> l) (k mod e)).
+    template <int R>
+    __device__ __forceinline__ void dit(u64 (&x)[16], u64 c) const {
+        constexpr int LGR = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : 1;
+        u64 pw[LGR];  // pw[l] = c^(R >> (l + 1))
+        pw[LGR - 1] = c;
+#pragma unroll
+        for (int l = LGR - 2; l >= 0; --l) pw[l] = mmul(pw[l + 1], pw[l + 1], m);
+#pragma unroll
+        for (int l = 0; l < LGR; ++l) {
+            const int e = 1 << l;
+            u64 tw[8];
+#pragma unroll
+            for (int r = 0; r < e; ++r) tw[r] = r == 0 ? pw[l] : mmul(pw[l], c16i[(8 >> l) * r], m);
+#pragma unroll
+            for (int k0 = 0; k0 < R; k0 += 2 * e)
+#pragma unroll
+                for (int r = 0; r < e; ++r) {
+                    const u64 u = x[k0 + r], o = mmul(x[k0 + r + e], tw[r], m);
+                    x[k0 + r] = madd(u, o, m);
+                    x[k0 + r + e] = msub(u, o, m);
+                }
+        }
+    }
+    // Forward passes P .. NF-2 (radix 16, h = L/2 / 16^P), one phase each.
+    template <int P>
+    __device__ __forceinline__ void fwd_mid() {
+        if constexpr (P <= NF - 2) {
+            constexpr int H = (L / 2) >> (4 * P), S = 2 * H / 16;
+            for (int k = t; k < G1; k += T) {
+                const int j = k & (S - 1), i0 = (k - j) * 16 + j;  // block (k / S) of 2H = 16 S entries
+                u64 x[16];
+                ld<16>(x, i0, S);
+                dif<16>(x, bw[j * (L / (2 * H))]);
+                st<16>(x, i0, S);
+            }
+            sync();
+            fwd_mid<P + 1>();
+        }
+    }
+    // Inverse passes P .. NF-2 (radix 16; h = RP 16^(P-1) after a partial first pass, else 16^P).
+    template <int P>
+    __device__ __forceinline__ void inv_mid() {
+        if constexpr (P <= NF - 2) {
+            constexpr int H = (LG % 4 ? PP::RP : 16) << (4 * (P - 1));
+            for (int k = t; k < G1; k += T) {
+                const int j = k & (H - 1), i0 = (k - j) * 16 + j;  // block (k / H) of 16 H entries
+                u64 x[16];
+                ld<16>(x, i0, H);
+                dit<16>(x, bwi[j * (L / (16 * H))]);
+                st<16>(x, i0, H);
+            }
+            sync();
+            inv_mid<P + 1>();
+        }
+    }
+    // The middle of a cyclic convolution with the kernel's spectrum, from the phase after the first
+    // forward pass (or, when L = 16, after the input alone) to the phase before the last inverse pass.
+    __device__ __forceinline__ void conv_mid() {
+        fwd_mid<1>();
+        // the last forward pass, the product with the spectrum and the first inverse pass share
+        // their groups (RM consecutive entries, j = 0)
+        for (int k = t; k < L / RM; k += T) {
+            u64 y[16];
+            ld<RM>(y, k * RM, 1);
+            dif<RM>(y, m.one);
+#pragma unroll
+            for (int i = 0; i < RM; ++i) y[i] = mmul(y[i], a.hat[k * RM + i], m);
+            dit<RM>(y, m.one);
+            st<RM>(y, k * RM, 1);
+        }
         sync();
-        inv();
+        inv_mid<1>();
     }
-    // entry (r, c), c < r, of a table: the triangle row when column block c/16 has been copied
-    // into the rows (it ends before row block r/16), else the staging column c % 16
-    __device__ u64 v_at(int r, int c) const {
-        return (c >> 4) < (r >> 4) ? a.Vt[(int64_t)r * (r - 1) / 2 + c] : a.Vs[(int64_t)(c & 15) * (d.nm + 1) + r];
+    // The first forward pass of group k (entries k + i G1) from registers into LDS.
+    __device__ __forceinline__ void first_pass(u64 (&x)[16], int k) {
+        if constexpr (NF >= 2) dif<16>(x, bw[k]);
+        st<16>(x, k, G1);
     }
-    __device__ u64 s_at(int r, int c) const {  // 1 <= c <= r
-        return (c >> 4) < (r >> 4) ? a.St[(int64_t)r * (r - 1) / 2 + c - 1] : a.Ss[(int64_t)(c & 15) * (d.nm + 1) + r];
+    // The last inverse pass of group k: x[i] = result entry k + i G1.
+    __device__ __forceinline__ void last_pass(u64 (&x)[16], int k) {
+        ld<16>(x, k, G1);
+        if constexpr (NF >= 2) dit<16>(x, bwi[k]);
     }
-    // stage-1 input of row v into buf
-    __device__ void stage1_in(int v) {
+    // Entry (r, c) of a table, row block r/16 the one whose segment slots are in segrow: its tile
+    // when column block c/16 has been flushed (it ends before row block r/16), else the staging
+    // column of c. Branch-free, so a phase's loads are all in flight together.
+    __device__ __forceinline__ int64_t tab_ix(int r, int c) const {
+        const int Y = r >> 4, X = c >> 4;
+        const int64_t tile = (int64_t)segrow[X >> 4] * 16 + (X & 15);
+        return X < Y ? tile * 256 + ((r & 15) << 4) + (c & 15) : st_col(c, d.NS) + r;
+    }
+    __device__ __forceinline__ u64 v_at(int r, int c) const {
+        return ((c >> 4) < (r >> 4) ? a.Vpool : a.Vs)[tab_ix(r, c)];
+    }
+    __device__ __forceinline__ u64 s_at(int r, int c) const {  // 1 <= c <= r
+        return ((c >> 4) < (r >> 4) ? a.Spool : a.Ss)[tab_ix(r, c)];
+    }
+    // stage-1 input a[w] = P2[v][w] / v, w < v
+    __device__ __forceinline__ u64 in1(int v, int w, u64 iv) const { return mmul(v_at(v, w), iv, m); }
+    // stage-1 input of group k into x (entries w = k + i G1)
+    __device__ __forceinline__ void in1_grp(u64 (&x)[16], int v, int k, u64 iv) const {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int w = k + i * G1;
+            const u64 y = v_at(v, w < v ? w : 0);
+            x[i] = w < v ? mmul(y, iv, m) : 0;
+        }
+    }
+    // stage-1 input of row v into buf (the last row's queries)
+    __device__ __forceinline__ void stage1_in(int v) {
         const u64 iv = a.inv[v];
-        for (int w = t; w < L; w += T) buf[w] = w < v ? mmul(v_at(v, w), iv, m) : 0;
+        for (int w = t; w < L; w += T) buf[sw(w)] = w < v ? in1(v, w, iv) : 0;
         sync();
+    }
+    // Adds the terms of the first cnt <= 8 queries over the entries x[i] (w = k + i G1) of row v.
+    __device__ __forceinline__ void query_add(u64 (&acc)[kQG], const u64 (&x)[16], int v, int k, int cnt) const {
+#pragma unroll
+        for (int jj = 0; jj < kQG; ++jj) {
+            if (jj < cnt) {  // cnt is uniform: whole waves take the same branches
+                const int M = q[d.q_off + jj].M;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int w = k + i * G1;
+                    acc[jj] = madd(acc[jj], mmul(x[i], a.inv[w < v ? M - w : 1], m), m);  // x[i] = 0 for w >= v
+                }
+            }
+        }
+    }
+    // The wave sums of acc into red[wave][jj].
+    __device__ __forceinline__ void query_wave(const u64 (&acc)[kQG], int cnt) {
+#pragma unroll
+        for (int jj = 0; jj < kQG; ++jj) {
+            if (jj < cnt) {
+                u64 s = acc[jj];
+                for (int o = 32; o >= 1; o >>= 1) s = madd(s, shfl_xor_u64(s, o), m);
+                if ((t & 63) == 0) red[(t >> 6) * kQG + jj] = s;
+            }
+        }
+    }
+    // Thread t < cnt adds the wave sums of query j0 + t and updates its Horner accumulator.
+    __device__ __forceinline__ void query_finish(int v, int j0, int cnt) {
+        if (t < cnt) {
+            u64 s = 0;
+            for (int wv = 0; wv < NW; ++wv) s = madd(s, red[wv * kQG + t], m);
+            const int j = j0 + t;
+            const QueryDesc& qq = q[d.q_off + j];
+            const u64 coef = mmul(mmul(a.fact[qq.n], a.ifact[qq.n - v], m), a.ifact[qq.M - v], m);
+            a.qtot[2 * j] = madd(mmul(a.qtot[2 * j], a.qtot[2 * j + 1], m), mmul(coef, s, m), m);
+        }
     }
     // query terms of row v (in buf) for the queries with n >= v
-    __device__ void queries(int v) {
+    __device__ __forceinline__ void queries(int v) {
         int nact = 0;
         while (nact < d.q_cnt && q[d.q_off + nact].n >= v) ++nact;
         for (int j0 = 0; j0 < nact; j0 += kQG) {
@@ -220,54 +375,56 @@ struct Work62 {
 #pragma unroll
             for (int jj = 0; jj < kQG; ++jj) acc[jj] = 0;
             for (int w = t; w < v; w += T) {
-                const u64 x = buf[w];
+                const u64 x = buf[sw(w)];
 #pragma unroll
                 for (int jj = 0; jj < kQG; ++jj)
                     if (jj < cnt) acc[jj] = madd(acc[jj], mmul(x, a.inv[q[d.q_off + j0 + jj].M - w], m), m);
             }
 #pragma unroll
             for (int jj = 0; jj < kQG; ++jj) {
-                if (jj < cnt) {  // cnt is uniform: whole waves take the same branches
+                if (jj < cnt) {
                     u64 s = acc[jj];
                     for (int o = 32; o >= 1; o >>= 1) s = madd(s, shfl_xor_u64(s, o), m);
                     if ((t & 63) == 0) red[(t >> 6) * kQG + jj] = s;
                 }
             }
             sync();
-            if (t < cnt) {
-                u64 s = 0;
-                for (int wv = 0; wv < NW; ++wv) s = madd(s, red[wv * kQG + t], m);
-                const int j = j0 + t;
-                const QueryDesc& qq = q[d.q_off + j];
-                const u64 coef = mmul(mmul(a.fact[qq.n], a.ifact[qq.n - v], m), a.ifact[qq.M - v], m);
-                a.qtot[2 * j] = madd(mmul(a.qtot[2 * j], a.qtot[2 * j + 1], m), mmul(coef, s, m), m);
-            }
+            query_finish(v, j0, cnt);
             sync();
         }
     }
-    // column block X (columns 16 X .. 16 X + 15) from staging into the triangles' rows
-    __device__ void flush(int X) {
-        const int nm = d.nm, NS = nm + 1, c0 = 16 * X;
-        for (int r = c0 + 1 + t; r <= nm; r += T) {  // P2 rows r > c
-            u64* row = a.Vt + (int64_t)r * (r - 1) / 2;
-            for (int k = 0; k < 16 && c0 + k < r; ++k) row[c0 + k] = a.Vs[(int64_t)k * NS + r];
+    // Column block X from staging into its tiles (Y, X) of both tables, for the row blocks Y > X,
+    // one row of 16 entries per thread and table; then the segment slots of row block X + 1 (the
+    // rows of the next 16 steps) into segrow.
+    __device__ __forceinline__ void flush(int X) {
+        const int NS = d.NS, r0 = 16 * (X + 1);
+        const int64_t st = st_col(16 * X, NS);
+        for (int i = t; i < 2 * (NS - r0); i += T) {
+            const int tab = i >= NS - r0, r = r0 + (tab ? i - (NS - r0) : i);
+            const u64* src = (tab ? a.Ss : a.Vs) + st + r;
+            const int64_t tile = (int64_t)slots[d.slot_off + (r >> 4) * d.NSEG + (X >> 4)] * 16 + (X & 15);
+            u64* dst = (tab ? a.Spool : a.Vpool) + tile * 256 + ((r & 15) << 4);
+            u64 y[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) y[k] = src[(int64_t)k * NS];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) dst[k] = y[k];
         }
-        for (int r = c0 + t; r < nm; r += T) {  // S2 rows r >= q, q >= 1
-            u64* row = a.St + (int64_t)r * (r - 1) / 2;
-            for (int k = 0; k < 16 && c0 + k <= r; ++k)
-                if (c0 + k >= 1) row[c0 + k - 1] = a.Ss[(int64_t)k * NS + r];
-        }
+        for (int s = t; s < d.NSEG; s += T) segrow[s] = slots[d.slot_off + (X + 1) * d.NSEG + s];
         sync();
     }
 
-    __device__ void run() {
-        const int nm = d.nm, NS = nm + 1, v0 = d.model ? 1 : d.dd;
-        // staging to zero (column 0 of P2 is never computed: V[p][0] = 0), query state, spectrum
-        for (int i = t; i < 32 * NS; i += T) a.Vs[i] = 0;
+    __device__ __forceinline__ void run() {
+        const int nm = d.nm, NS = d.NS, v0 = d.model ? 1 : d.dd;
+        // staging to zero (column 0 of P2 is never computed: V[p][0] = 0), query state, spectrum,
+        // the twiddle tables of the register passes
+        for (int i = t; i < 64 * NS; i += T) a.Vs[i] = 0;
         for (int j = t; j < d.q_cnt; j += T) {
             a.qtot[2 * j] = 0;
             a.qtot[2 * j + 1] = g_of(q[d.q_off + j].M);
         }
+        for (int i = t; i < G1; i += T) bw[i] = a.tw[L / 2 + i], bwi[i] = a.twi[L / 2 + i];  // w^i = tw[L/2 + i]
+        if (t < 8) c16[t] = a.tw[8 + t], c16i[t] = a.twi[8 + t];                          // w_16^r = tw[8 + r]
         for (int i = t; i < L; i += T) buf[i] = (i >= 1 && i < nm) ? a.inv[i] : 0;
         sync();
         fwd();
@@ -275,31 +432,95 @@ struct Work62 {
         sync();
         for (int v = 1; v < nm; ++v) {
             const int col = v & 15;
-            u64* scol = a.Ss + (int64_t)col * NS;
-            u64* vcol = a.Vs + (int64_t)col * NS;
+            u64* scol = a.Ss + st_col(v, NS);
+            u64* vcol = a.Vs + st_col(v, NS);
+            int nact = 0;
+            while (nact < d.q_cnt && q[d.q_off + nact].n >= v) ++nact;
             if (v > v0) {
-                stage1_in(v);
-                queries(v);
-                convolve();
-                for (int e = v + t; e < nm; e += T) scol[e] = mmul(buf[e], a.ifact2[e - v], m);
+                // stage-1 input of row v, the query sums, the first forward pass; g's power tables
+                const u64 iv = a.inv[v];
+                power_tables(g_of(v));
+                if (nact > kQG) {  // many queries: their sums from the input in LDS first
+                    for (int k = t; k < G1; k += T) {
+                        u64 x[16];
+                        in1_grp(x, v, k, iv);
+                        st<16>(x, k, G1);
+                    }
+                    sync();
+                    queries(v);
+                    for (int k = t; k < G1; k += T) {
+                        u64 x[16];
+                        ld<16>(x, k, G1);
+                        first_pass(x, k);
+                    }
+                } else {
+                    u64 acc[kQG];
+#pragma unroll
+                    for (int jj = 0; jj < kQG; ++jj) acc[jj] = 0;
+                    for (int k = t; k < G1; k += T) {
+                        u64 x[16];
+                        in1_grp(x, v, k, iv);
+                        query_add(acc, x, v, k, nact);
+                        first_pass(x, k);
+                    }
+                    query_wave(acc, nact);  // finished after the convolution
+                }
+                sync();
+                conv_mid();
+                // last inverse pass; stage-1 output S2[e][v] = c[e] / (e - v)!^2, v <= e < nm
+                for (int k = t; k < G1; k += T) {
+                    u64 x[16];
+                    last_pass(x, k);
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const int e = k + i * G1;
+                        if (e >= v && e < nm) scol[e] = mmul(x[i], a.ifact2[e - v], m);
+                    }
+                }
+                if (nact > 0 && nact <= kQG) query_finish(v, 0, nact);
             } else {
                 for (int e = v + t; e < nm; e += T) scol[e] = 0;
+                if (v == v0) power_tables(g_of(v));
             }
             sync();
             if (v >= v0) {
-                power_tables(g_of(v));
-                const bool conv = v > v0;
-                if (conv) {
-                    const u64 fv = a.fact[v];
-                    for (int i = t; i < L; i += T) buf[i] = (i >= 1 && i <= v) ? mmul(mmul(s_at(v, i), fv, m), gpow(v - i), m) : 0;
-                    sync();
-                    convolve();
-                }
                 const u64 iv = a.ifact[v], s1 = mmul(gpow(v), iv, m);
-                for (int p = v + 1 + t; p <= nm; p += T) {
-                    const u64 pc = conv ? mmul(a.mont[p], buf[p & (L - 1)], m) : 0;
-                    const u64 f = mmul(mmul(gpow(p - v), iv, m), a.ifact2[p - v - 1], m);
-                    vcol[p] = mmul(msub(s1, pc, m), f, m);
+                if (v > v0) {
+                    // stage-2 input b[i] = S2[v][i] v! g^(v-i), 1 <= i <= v; the first forward pass
+                    const u64 fv = a.fact[v];
+                    for (int k = t; k < G1; k += T) {
+                        u64 x[16];
+#pragma unroll
+                        for (int i = 0; i < 16; ++i) {
+                            const int e = k + i * G1;
+                            const bool ok = e >= 1 && e <= v;
+                            const int ce = ok ? e : 1;
+                            const u64 y = s_at(v, ce);
+                            x[i] = ok ? mmul(mmul(y, fv, m), gpow(v - ce), m) : 0;
+                        }
+                        first_pass(x, k);
+                    }
+                    sync();
+                    conv_mid();
+                    // last inverse pass; stage-2 output P2[p][v], v < p <= nm, from entry p mod L
+                    for (int k = t; k < G1; k += T) {
+                        u64 x[16];
+                        last_pass(x, k);
+#pragma unroll
+                        for (int i = 0; i < 16; ++i) {
+                            const int e = k + i * G1, p = e == 0 ? L : e;
+                            if (p > v && p <= nm) {
+                                const u64 pc = mmul(a.mont[p], x[i], m);
+                                const u64 f = mmul(mmul(gpow(p - v), iv, m), a.ifact2[p - v - 1], m);
+                                vcol[p] = mmul(msub(s1, pc, m), f, m);
+                            }
+                        }
+                    }
+                } else {
+                    for (int p = v + 1 + t; p <= nm; p += T) {
+                        const u64 f = mmul(mmul(gpow(p - v), iv, m), a.ifact2[p - v - 1], m);
+                        vcol[p] = mmul(s1, f, m);
+                    }
                 }
             } else {
                 for (int p = v + 1 + t; p <= nm; p += T) vcol[p] = 0;
@@ -316,13 +537,16 @@ struct Work62 {
 };
 
 template <int LG>
-__global__ void __launch_bounds__(kP62Threads)
-    k_p62(const Task62* __restrict__ tasks, const QueryDesc* __restrict__ qs, u64* __restrict__ mem, u64* __restrict__ res) {
+__global__ void __launch_bounds__(P62<LG>::T)
+    k_p62(const Task62* __restrict__ tasks, const QueryDesc* __restrict__ qs, const int* __restrict__ slots,
+          u64* __restrict__ mem, u64* __restrict__ res) {
     using PP = P62<LG>;
     __shared__ u64 buf[PP::L];
     __shared__ u64 glo[64], ghi[PP::HI], red[PP::NW * kQG];
+    __shared__ u64 bw[PP::G1], bwi[PP::G1], c16[8], c16i[8];
+    __shared__ int segrow[PP::NSEG];
     const Task62& d = tasks[blockIdx.x];
-    Work62<LG> w{d, d.m, arena62(mem, d), qs, buf, glo, ghi, red, (int)threadIdx.x};
+    Work62<LG> w{d, d.m, arena62(mem, d), qs, buf, glo, ghi, red, bw, bwi, c16, c16i, slots, segrow, (int)threadIdx.x};
     w.run();
     for (int j = threadIdx.x; j < d.q_cnt; j += PP::T) {
         const QueryDesc& qq = qs[d.q_off + j];
@@ -331,32 +555,33 @@ __global__ void __launch_bounds__(kP62Threads)
 }
 
 template <int LG>
-hipError_t launch_lg(int ntasks, const Task62* tasks, const QueryDesc* qs, u64* mem, u64* res, hipStream_t stream) {
-    k_p62<LG><<<dim3((unsigned)ntasks), P62<LG>::T, 0, stream>>>(tasks, qs, mem, res);
+hipError_t launch_lg(int ntasks, const Task62* tasks, const QueryDesc* qs, const int* slots, u64* mem, u64* res,
+                     hipStream_t stream) {
+    k_p62<LG><<<dim3((unsigned)ntasks), P62<LG>::T, 0, stream>>>(tasks, qs, slots, mem, res);
     return hipGetLastError();
 }
 
 }  // namespace
 
-int64_t arena62_words(int nm, int F1, int L, int q_cnt) {
-    return 5 * (int64_t)F1 + 3 * (int64_t)L + (int64_t)nm * (nm + 1) + 32 * (int64_t)(nm + 1) + 2 * (int64_t)q_cnt;
+int64_t arena62_words(int NS, int64_t pool, int F1, int L, int q_cnt) {
+    return 5 * (int64_t)F1 + 3 * (int64_t)L + 512 * pool + 64 * (int64_t)NS + 2 * (int64_t)q_cnt;
 }
 
-hipError_t launch_p62(int log2L, int ntasks, const Task62* tasks, const QueryDesc* qs, uint64_t* mem, uint64_t* res,
-                      hipStream_t stream) {
+hipError_t launch_p62(int log2L, int ntasks, const Task62* tasks, const QueryDesc* qs, const int* slots, uint64_t* mem,
+                      uint64_t* res, hipStream_t stream) {
     if (ntasks <= 0) return hipSuccess;
     switch (log2L) {
-        case 4: return launch_lg<4>(ntasks, tasks, qs, mem, res, stream);
-        case 5: return launch_lg<5>(ntasks, tasks, qs, mem, res, stream);
-        case 6: return launch_lg<6>(ntasks, tasks, qs, mem, res, stream);
-        case 7: return launch_lg<7>(ntasks, tasks, qs, mem, res, stream);
-        case 8: return launch_lg<8>(ntasks, tasks, qs, mem, res, stream);
-        case 9: return launch_lg<9>(ntasks, tasks, qs, mem, res, stream);
-        case 10: return launch_lg<10>(ntasks, tasks, qs, mem, res, stream);
-        case 11: return launch_lg<11>(ntasks, tasks, qs, mem, res, stream);
-        case 12: return launch_lg<12>(ntasks, tasks, qs, mem, res, stream);
-        case 13: return launch_lg<13>(ntasks, tasks, qs, mem, res, stream);
-        case 14: return launch_lg<14>(ntasks, tasks, qs, mem, res, stream);
+        case 4: return launch_lg<4>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 5: return launch_lg<5>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 6: return launch_lg<6>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 7: return launch_lg<7>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 8: return launch_lg<8>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 9: return launch_lg<9>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 10: return launch_lg<10>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 11: return launch_lg<11>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 12: return launch_lg<12>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 13: return launch_lg<13>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 14: return launch_lg<14>(ntasks, tasks, qs, slots, mem, res, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -22,18 +22,20 @@ struct Mod62 {
 
 struct Task62 {
     Mod62 m;
-    uint64_t linv;      // 1/L, Montgomery form
-    int64_t arena;      // offset (words) of the task's memory
-    int nm, F1, L;      // table rows 0..nm, factorials 0..F1-1, transform length
-    int q_off, q_cnt;   // queries, sorted by n (largest first)
+    uint64_t linv;       // 1/L, Montgomery form
+    int64_t arena;       // offset (words) of the task's memory
+    int nm, F1, L, NS;   // table rows 0..nm, factorials 0..F1-1, transform length, staging rows
+    int q_off, q_cnt;    // queries, sorted by n (largest first)
     int dd, model;
-    int pad_;
+    int slot_off, NSEG;  // the tile-slot plan of nm (plan_slots): offset in the launch's slot array, segments per row block
+    int64_t pool;        // tiles per table
 };
 
-// Words of one task's arena: five factorial-range tables, three of length L, the two packed
-// triangles, the staging of both tables and the query state.
-int64_t arena62_words(int nm, int F1, int L, int q_cnt);
-hipError_t launch_p62(int log2L, int ntasks, const Task62* tasks, const QueryDesc* qs, uint64_t* mem, uint64_t* res,
-                      hipStream_t stream);
+// Words of one task's arena: five factorial-range tables, three of length L, the two tile pools
+// (pool tiles of 256 words each), the staging of both tables (32 columns of NS rows) and the
+// query state.
+int64_t arena62_words(int NS, int64_t pool, int F1, int L, int q_cnt);
+hipError_t launch_p62(int log2L, int ntasks, const Task62* tasks, const QueryDesc* qs, const int* slots, uint64_t* mem,
+                      uint64_t* res, hipStream_t stream);
 
 }  // namespace cf

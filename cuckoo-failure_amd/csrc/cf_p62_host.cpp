@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <string>
 #include <thread>
 #include <vector>
@@ -54,6 +55,7 @@ struct Plan62 {
     u64 p;
     int nm, F1, lg;
     std::vector<QueryDesc> q;  // sorted by n, largest first
+    const SlotPlan* sp;        // tile slots of the tables (shared by the tasks of one nm)
     int64_t words;
 };
 
@@ -104,12 +106,14 @@ namespace cf_int {
 // Device bytes of one such task (for the drivers' batch sizing).
 size_t p62_task_bytes(int nmax, int Mmax, size_t J) {
     const int F1 = std::max(nmax, Mmax) + 1;
-    return 8 * (size_t)arena62_words(nmax, F1, 1 << log2_length62(nmax), (int)J);
+    const SlotPlan sp = plan_slots(nmax);
+    return 8 * (size_t)arena62_words(sp.NS, sp.pool, F1, 1 << log2_length62(nmax), (int)J);
 }
 
 int run_p62(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, size_t J, const uint64_t* primes,
             const std::vector<size_t>& idx, const uint8_t* mask, uint64_t* res) {
     std::vector<Plan62> plans;
+    std::map<int, SlotPlan> slot_plans;  // by nm
     for (size_t b : idx) {
         Plan62 pl;
         pl.prime = b;
@@ -134,7 +138,10 @@ int run_p62(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, 
                                                  std::to_string(pl.nm));
         std::stable_sort(pl.q.begin(), pl.q.end(), [](const QueryDesc& a, const QueryDesc& c) { return a.n > c.n; });
         pl.F1 = std::max(pl.nm, Mmax) + 1;
-        pl.words = arena62_words(pl.nm, pl.F1, 1 << pl.lg, (int)pl.q.size());
+        auto it = slot_plans.find(pl.nm);
+        if (it == slot_plans.end()) it = slot_plans.emplace(pl.nm, plan_slots(pl.nm)).first;
+        pl.sp = &it->second;
+        pl.words = arena62_words(pl.sp->NS, pl.sp->pool, pl.F1, 1 << pl.lg, (int)pl.q.size());
         plans.push_back(std::move(pl));
     }
     // launches: tasks of one transform length, as many as fit in the device memory
@@ -153,6 +160,8 @@ int run_p62(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, 
         const size_t k = end - at;
         std::vector<Task62> td(k);
         std::vector<QueryDesc> qd;
+        std::vector<int> slots;            // the slot plans of this launch's table sizes
+        std::map<int, int> slot_at;        // nm -> offset in slots
         std::vector<size_t> table_off(k);
         size_t table_words = 0;
         int64_t arena = 0;
@@ -163,6 +172,15 @@ int run_p62(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, 
             td[i].q_cnt = (int)pl.q.size();
             td[i].dd = d;
             td[i].model = model;
+            auto so = slot_at.find(pl.nm);
+            if (so == slot_at.end()) {
+                so = slot_at.emplace(pl.nm, (int)slots.size()).first;
+                slots.insert(slots.end(), pl.sp->segs.begin(), pl.sp->segs.end());
+            }
+            td[i].slot_off = so->second;
+            td[i].NSEG = pl.sp->NSEG;
+            td[i].NS = pl.sp->NS;
+            td[i].pool = pl.sp->pool;
             for (size_t j = 0; j < pl.q.size(); ++j) {
                 QueryDesc qq = pl.q[j];
                 qq.out = (int64_t)qd.size();  // this launch's result slot
@@ -183,13 +201,15 @@ int run_p62(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, 
         for (auto& th : ths) th.join();
         int rc = ensure(e, &e->arena, &e->arena_cap, bytes, "the 62-bit tables");
         if (rc) return rc;
-        const size_t small = sizeof(Task62) * k + sizeof(QueryDesc) * qd.size() + sizeof(u64) * qd.size() + 512;
+        const size_t small = sizeof(Task62) * k + sizeof(QueryDesc) * qd.size() + sizeof(u64) * qd.size() +
+                             sizeof(int) * slots.size() + 1024;
         rc = ensure(e, &e->small, &e->small_cap, small, "the 62-bit descriptors");
         if (rc) return rc;
         char* sp = (char*)e->small;
         Task62* dtd = (Task62*)sp;
         QueryDesc* dqd = (QueryDesc*)(sp + ((sizeof(Task62) * k + 255) & ~(size_t)255));
         u64* dres = (u64*)((char*)dqd + ((sizeof(QueryDesc) * qd.size() + 255) & ~(size_t)255));
+        int* dslots = (int*)((char*)dres + ((sizeof(u64) * qd.size() + 255) & ~(size_t)255));
         u64* mem = (u64*)e->arena;
         for (size_t i = 0; i < k; ++i) {
             const size_t w = 5 * (size_t)plans[at + i].F1 + 2 * ((size_t)1 << plans[at + i].lg);
@@ -198,8 +218,10 @@ int run_p62(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, 
         }
         HIP_TRY(e, hipMemcpyAsync(dtd, td.data(), sizeof(Task62) * k, hipMemcpyHostToDevice, e->stream), "upload");
         HIP_TRY(e, hipMemcpyAsync(dqd, qd.data(), sizeof(QueryDesc) * qd.size(), hipMemcpyHostToDevice, e->stream), "upload");
+        HIP_TRY(e, hipMemcpyAsync(dslots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, e->stream),
+                "upload");
         HIP_TRY(e, hipEventRecord(e->ev0, e->stream), "hipEventRecord");
-        HIP_TRY(e, launch_p62(plans[at].lg, (int)k, dtd, dqd, mem, dres, e->stream), "62-bit launch");
+        HIP_TRY(e, launch_p62(plans[at].lg, (int)k, dtd, dqd, dslots, mem, dres, e->stream), "62-bit launch");
         HIP_TRY(e, hipEventRecord(e->ev1, e->stream), "hipEventRecord");
         out.assign(qd.size(), 0);
         HIP_TRY(e, hipMemcpyAsync(out.data(), dres, sizeof(u64) * qd.size(), hipMemcpyDeviceToHost, e->stream), "download");
