@@ -16,7 +16,8 @@
 //                      P2[p][v] = (g^v/v! - p c[p mod L]) g^(p-v)/v! / (p-v-1)!^2,  v < p <= nm
 //   queries:           tot_j <- tot_j Q_j + n_j!/((n_j-v)! (M_j-v)!) sum_w a[w] / (M_j - w)   (n_j >= v)
 // with k[t] = 1/t (1 <= t < nm) and the result N_bad = M_j! tot_j. One workgroup runs one prime:
-// the cyclic convolution of length L = max(16, next_pow2(nm)) <= 2^14 in LDS (DIF forward, natural
+// the cyclic convolution of length L = max(16, next_pow2(nm)) <= 2^17, in LDS up to 2^14 and in the
+// task's HBM arena beyond (every pass still loads and stores whole groups; DIF forward, natural
 // in and bit-reversed out; pointwise product with the kernel's spectrum, made by the same forward
 // and scaled by 1/L; DIT inverse back to natural order; radix-16 register passes). The tables are
 // kept as the main engine keeps them (cf_core.hpp tab_off, cf_host.cpp plan_slots): 16 x 16 tiles
@@ -70,14 +71,14 @@ constexpr int kQG = 8;  // queries reduced together
 
 // The transform of length L = 2^LG as register passes: F16 = LG / 4 passes of radix 16 and, when
 // LG % 4 = r > 0, one pass of radix RP = 2^r (the last forward pass, the first inverse pass).
-// A radix-16 pass has G1 = L / 16 groups; thread t holds groups t, t + T, .. (two at L = 2^14: 512
-// threads with 256 registers each, rather than 1024 with 128).
+// A radix-16 pass has G1 = L / 16 groups; thread t holds groups t, t + T, .. (at most 512 threads,
+// so each has up to 256 registers: two groups each at L = 2^14, eight at 2^17).
 template <int LG>
 struct P62 {
     static constexpr int L = 1 << LG;
     static constexpr int G1 = L / 16;
-    static constexpr int VT = LG == 14 ? 2 : 1;
-    static constexpr int T = G1 / VT < 64 ? 64 : G1 / VT;  // whole waves
+    static constexpr int T = G1 < 64 ? 64 : (G1 < 512 ? G1 : 512);  // whole waves
+    static constexpr bool HBM = LG > 14;  // the transform's buffer in the arena, not in LDS
     static constexpr int NW = T / 64;
     static constexpr int HI = L / 64 + 2;
     static constexpr int F16 = LG / 4, RP = 1 << (LG % 4), NF = F16 + (LG % 4 ? 1 : 0);
@@ -94,6 +95,7 @@ struct Arena62 {
     u64 *Vpool, *Spool;                       // tiles of 16 x 16 entries, row-major
     u64 *Vs, *Ss;                             // staging: 2 blocks x 16 columns x NS rows, column-major
     u64* qtot;                                // 2 per query: Horner accumulator, Q = g(M)
+    u64* tbuf;                                // L: the transform's buffer when it is not in LDS
 };
 __device__ __forceinline__ Arena62 arena62(u64* mem, const Task62& d) {
     Arena62 a;
@@ -111,6 +113,7 @@ __device__ __forceinline__ Arena62 arena62(u64* mem, const Task62& d) {
     a.Vs = a.Spool + tiles;
     a.Ss = a.Vs + 32 * NS;
     a.qtot = a.Ss + 32 * NS;
+    a.tbuf = a.qtot + 2 * (int64_t)d.q_cnt;
     return a;
 }
 
@@ -541,12 +544,14 @@ __global__ void __launch_bounds__(P62<LG>::T)
     k_p62(const Task62* __restrict__ tasks, const QueryDesc* __restrict__ qs, const int* __restrict__ slots,
           u64* __restrict__ mem, u64* __restrict__ res) {
     using PP = P62<LG>;
-    __shared__ u64 buf[PP::L];
+    __shared__ u64 lbuf[PP::HBM ? 1 : PP::L];
     __shared__ u64 glo[64], ghi[PP::HI], red[PP::NW * kQG];
     __shared__ u64 bw[PP::G1], bwi[PP::G1], c16[8], c16i[8];
     __shared__ int segrow[PP::NSEG];
     const Task62& d = tasks[blockIdx.x];
-    Work62<LG> w{d, d.m, arena62(mem, d), qs, buf, glo, ghi, red, bw, bwi, c16, c16i, slots, segrow, (int)threadIdx.x};
+    const Arena62 ar = arena62(mem, d);
+    u64* buf = PP::HBM ? ar.tbuf : lbuf;
+    Work62<LG> w{d, d.m, ar, qs, buf, glo, ghi, red, bw, bwi, c16, c16i, slots, segrow, (int)threadIdx.x};
     w.run();
     for (int j = threadIdx.x; j < d.q_cnt; j += PP::T) {
         const QueryDesc& qq = qs[d.q_off + j];
@@ -564,7 +569,8 @@ hipError_t launch_lg(int ntasks, const Task62* tasks, const QueryDesc* qs, const
 }  // namespace
 
 int64_t arena62_words(int NS, int64_t pool, int F1, int L, int q_cnt) {
-    return 5 * (int64_t)F1 + 3 * (int64_t)L + 512 * pool + 64 * (int64_t)NS + 2 * (int64_t)q_cnt;
+    return 5 * (int64_t)F1 + 3 * (int64_t)L + 512 * pool + 64 * (int64_t)NS + 2 * (int64_t)q_cnt +
+           (L > (1 << 14) ? (int64_t)L : 0);
 }
 
 hipError_t launch_p62(int log2L, int ntasks, const Task62* tasks, const QueryDesc* qs, const int* slots, uint64_t* mem,
@@ -582,6 +588,9 @@ hipError_t launch_p62(int log2L, int ntasks, const Task62* tasks, const QueryDes
         case 12: return launch_lg<12>(ntasks, tasks, qs, slots, mem, res, stream);
         case 13: return launch_lg<13>(ntasks, tasks, qs, slots, mem, res, stream);
         case 14: return launch_lg<14>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 15: return launch_lg<15>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 16: return launch_lg<16>(ntasks, tasks, qs, slots, mem, res, stream);
+        case 17: return launch_lg<17>(ntasks, tasks, qs, slots, mem, res, stream);
         default: return hipErrorInvalidValue;
     }
 }

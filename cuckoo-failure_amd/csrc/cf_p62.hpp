@@ -10,7 +10,7 @@
 namespace cf {
 
 constexpr int kP62Threads = 1024;  // threads of the longest transform's workgroup (L = 2^14)
-constexpr int kP62MaxLog2L = 14;   // transforms in LDS: tables of up to 16384 rows
+constexpr int kP62MaxLog2L = 17;   // tables of up to 131072 rows (transforms in LDS up to 2^14, in HBM beyond)
 
 // Montgomery arithmetic modulo an odd p < 2^63 with R = 2^64.
 struct Mod62 {
@@ -32,8 +32,8 @@ struct Task62 {
 };
 
 // Words of one task's arena: five factorial-range tables, three of length L, the two tile pools
-// (pool tiles of 256 words each), the staging of both tables (32 columns of NS rows) and the
-// query state.
+// (pool tiles of 256 words each), the staging of both tables (32 columns of NS rows), the query
+// state and the transform's buffer when it does not fit in LDS (L > 2^14).
 int64_t arena62_words(int NS, int64_t pool, int F1, int L, int q_cnt);
 hipError_t launch_p62(int log2L, int ntasks, const Task62* tasks, const QueryDesc* qs, const int* slots, uint64_t* mem,
                       uint64_t* res, hipStream_t stream);

@@ -132,7 +132,7 @@ int run_p62(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, 
         pl.lg = log2_length62(pl.nm);
         if (pl.lg > kP62MaxLog2L)
             return fail(e, CF_E_UNSUPPORTED, "n = " + std::to_string(pl.nm) + " modulo a prime above 2^50 needs a transform "
-                                             "longer than 2^14, which this build does not run");
+                                             "longer than 2^" + std::to_string(kP62MaxLog2L) + ", which this build does not run");
         if (two_adic_valuation(pl.p - 1) < pl.lg)
             return fail(e, CF_E_UNSUPPORTED, "p - 1 of " + std::to_string(pl.p) + " has too few factors of two for n = " +
                                                  std::to_string(pl.nm));

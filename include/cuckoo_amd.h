@@ -75,7 +75,7 @@ int cf_set_graphs(cf_engine* e, int on);
  *            g(v) = v^d, Q = M^d (single_table.hpp:16-23)
  *   n, M     J queries (1 <= n < M)
  *   primes   np odd primes p < 2^63 (each with M < p); those above 2^50 (the reference's 62-bit
- *            list of its -DFP_RB=64 build) run on a 64-bit Montgomery kernel, n <= 16384
+ *            list of its -DFP_RB=64 build) run on a 64-bit Montgomery kernel, n <= 131072
  *   mask     np x J bytes, row-major; query j is evaluated for prime b iff mask[b J + j] != 0.
  *            NULL means all.
  *   res      np x J canonical residues N_bad(n_j, M_j) mod p_b; 0 where the mask is 0.
@@ -267,8 +267,8 @@ long cf_max_batch(cf_engine* e, int nmax, int Mmax, size_t J);
 size_t cf_mem_per_prime(int nmax, int Mmax, size_t J);
 /* The same for primes above 2^50 (the reference's -DFP_RB=64 build with 62-bit primes,
  * fast/cuckoo_fast.cu:48-60, batch sizing :742-750): cf_run_fused runs such primes (odd, up to
- * 2^63, tables of up to 16384 rows) on a 64-bit Montgomery kernel, as packed triangles of
- * about 8 n^2 bytes per prime. */
+ * 2^63, tables of up to 131072 rows) on a 64-bit Montgomery kernel, in tile pools of about
+ * 4 n^2 bytes per prime. */
 long cf_max_batch62(cf_engine* e, int nmax, int Mmax, size_t J);
 size_t cf_mem_per_prime62(int nmax, int Mmax, size_t J);
 

@@ -17,6 +17,7 @@ the programs' outputs are kept, as data.
     python tests/golden/make_golden.py --only uneven_big # a sample of the uneven batch with n > 512
     python tests/golden/make_golden.py --only big        # n = 65536: primes 1..3, and 126 queries on 0..1
     python tests/golden/make_golden.py --only fast62     # the reference's -DFP_RB=64 build (62-bit primes)
+    python tests/golden/make_golden.py --only fast62wide # the same build on tables beyond 16384 rows
     python tests/golden/make_golden.py --only big17      # n = 70000 (transforms of 2^17), prime 0 (~39 GB, ~4 min)
 """
 import json
@@ -451,6 +452,31 @@ def fast62_cases():
         return list(ex.map(one, cases))
 
 
+def fast62_wide_cases():
+    """The reference's -DFP_RB=64 build on tables of more than 16384 rows (transforms of 2^15 and
+    2^16), one prime per process: several queries per table, d = 2, 3, 4."""
+    fast62 = os.path.join(REF, "cuckoo_fast62")
+    cases = [(3, [(16385, 16386), (20000, 40000), (17000, 20000)], 0, 1),
+             (3, [(16385, 16386), (20000, 40000), (17000, 20000)], 1, 1),
+             (2, [(16500, 17000)], 2, 1),
+             (4, [(16400, 16401)], 3, 1),
+             (3, [(32768, 65536)], 0, 1),
+             (3, [(40000, 80000)], 1, 1)]
+
+    def one(c):
+        d, qs, first, count = c
+        with tempfile.TemporaryDirectory() as td:
+            qf = os.path.join(td, "q.txt")
+            with open(qf, "w") as f:
+                f.write("".join(f"{n} {M}\n" for n, M in qs))
+            rc, so, se = run([fast62, str(d), qf, str(first), str(count)])
+        assert rc == 0, se
+        return {"d": d, "queries": qs, "first": first, "count": count, "stdout": so}
+
+    with ThreadPoolExecutor(max_workers=len(cases)) as ex:
+        return list(ex.map(one, cases))
+
+
 def _write(name, data):
     """Writes one fixture file after its data is complete (an interrupted run leaves the old file)."""
     tmp = os.path.join(HERE, name + ".tmp")
@@ -488,6 +514,9 @@ def main():
         return
     if only == "fast62":
         _write("fast62_cases.json", fast62_cases())
+        return
+    if only == "fast62wide":
+        _write("fast62_wide.json", fast62_wide_cases())
         return
     if only == "separate":
         _write("fail_separate_cases.json", fail_separate_cases())

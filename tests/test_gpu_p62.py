@@ -124,8 +124,38 @@ def test_cuckoo_fast62_prints_the_reference_lines():
         assert r.stdout == rec["stdout"], (rec["d"], rec["queries"])
 
 
+def _wide_groups():
+    # the reference's lines for tables of more than 16384 rows, grouped by (d, queries)
+    try:
+        recs = load("fast62_wide.json")
+    except OSError:
+        return []
+    groups = {}
+    for r in recs:
+        key = (r["d"], tuple(tuple(q) for q in r["queries"]))
+        groups.setdefault(key, []).extend(residue_lines(r["stdout"]))
+    return sorted(groups.items(), key=lambda g: max(n for n, _ in g[0][1]))
+
+
 @pytest.mark.gpu
-def test_62_bit_beyond_16384_rows_is_refused(eng):
+@pytest.mark.parametrize("g", range(5))
+def test_62_bit_tables_beyond_16384_rows(eng, g):
+    # transforms of 2^15 and 2^16 with the buffer in HBM: every line of the reference's
+    # -DFP_RB=64 build for tables of 16385 to 40000 rows
+    groups = _wide_groups()
+    if g >= len(groups):
+        pytest.skip("fixture not generated")
+    (d, qs), lines = groups[g]
+    idx = sorted({i for i, _, _, _, _ in lines})
+    primes = [p for i, p, _, _, _ in sorted(set((i, p, 0, 0, 0) for i, p, _, _, _ in lines))]
+    print(f"d={d} queries={qs} primes={idx}", flush=True)
+    got = eng.run(list(qs), primes, d)
+    for i, p, n, M, r in lines:
+        assert got[idx.index(i)][list(qs).index((n, M))] == r, (d, i, n, M)
+
+
+@pytest.mark.gpu
+def test_62_bit_beyond_131072_rows_is_refused(eng):
     with pytest.raises(cf.CfError) as ei:
-        eng.run([(16385, 16386)], P62[:1], 3)
+        eng.run([(131073, 131074)], P62[:1], 3)
     assert ei.value.code == cf.CF_E_UNSUPPORTED
