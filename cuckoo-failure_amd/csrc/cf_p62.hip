@@ -127,7 +127,8 @@ struct Work62 {
     const QueryDesc* q;
     u64* buf;
     u64 *glo, *ghi, *red;
-    u64 *bw, *bwi, *c16, *c16i;  // LDS: w^e and w^-e (e < G1) for w of order L; w_16^r and w_16^-r (r < 8)
+    u64 *W, *Wi, *wsm, *wsmi, *c16, *c16i;  // LDS: w^(16x), w^-(16x) (x < G1), w^b, w^-b (b < 16) for w of
+                                            // order L; w_16^r, w_16^-r (r < 8)
     const int* slots;            // segment slots of the plan, Y * NSEG + s
     int* segrow;                 // LDS: the segment slots of the current row block
     int t;
@@ -177,6 +178,10 @@ struct Work62 {
         }
     }
 
+    // w^e (tw) or w^-e (twi) for w of order L, e < L: tw[L/2 + e] = w^e below L/2, and w^(L/2) = -1
+    __device__ __forceinline__ u64 wexp(const u64* tab, int e) const {
+        return e < L / 2 ? tab[L / 2 + e] : msub(0, tab[e], m);
+    }
     // ---- the v-loop's transforms: groups of R entries in registers, one LDS round trip per pass ----
     // LDS word of transform entry i: bits 0-4 XORed with bits 4-8 (a bijection of [0, L)), so the
     // strided groups of every pass spread over the banks.
@@ -191,82 +196,101 @@ struct Work62 {
 #pragma unroll
         for (int k = 0; k < R; ++k) buf[sw(i0 + k * s)] = x[k];
     }
-    // Radix-R DIF over the levels h, h/2, .., 2h/R of the group x[k] = entry i0 + k (2h/R), i0 = j mod
-    // (2h/R) within its block of 2h, with b = w_2h^j: level l pairs (k, k + e), e = R / 2^(l+1), and
-    // multiplies the difference by w_(2h/2^l)^(j + (k mod e) 2h/R) = b^(2^l) w_16^((16/R) 2^l (k mod e)).
-    // The same butterflies, products and order as the radix-2 levels of fwd().
+    // A radix-R pass over the group x[k] = entry i0 + k S (i0 = j mod S within its block) does the
+    // butterflies of log2 R radix-2 levels. Their twiddles factor into constants (roots of order
+    // 16) and one power of w_L per entry: a DIF pass is the constant DIF network of R entries, then
+    // entry k times w_2h^(j rev(k)) (rev: the log2 R bits of k reversed); a DIT pass is entry k
+    // times w_(Rh)^-(j rev(k)), then the constant DIT network. Same products as the radix-2 levels,
+    // word for word.
+    // Constant DIF network: level l pairs (k, k + e), e = R / 2^(l+1), the difference times
+    // w_16^((16/R) 2^l r) for r = k mod e (none for r = 0).
     template <int R>
-    __device__ __forceinline__ void dif(u64 (&x)[16], u64 b) const {
-        constexpr int LGR = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : 1;
+    __device__ __forceinline__ void dif_c(u64 (&x)[16]) const {
 #pragma unroll
-        for (int l = 0; l < LGR; ++l) {
-            const int e = R >> (l + 1);
-            u64 tw[8];
-#pragma unroll
-            for (int r = 0; r < e; ++r) tw[r] = r == 0 ? b : mmul(b, c16[((16 / R) << l) * r], m);
+        for (int l = 0, e = R / 2; e >= 1; ++l, e >>= 1)
 #pragma unroll
             for (int k0 = 0; k0 < R; k0 += 2 * e)
 #pragma unroll
                 for (int r = 0; r < e; ++r) {
-                    const u64 u = x[k0 + r], w = x[k0 + r + e];
+                    const u64 u = x[k0 + r], w = x[k0 + r + e], df = msub(u, w, m);
                     x[k0 + r] = madd(u, w, m);
-                    x[k0 + r + e] = mmul(msub(u, w, m), tw[r], m);
+                    x[k0 + r + e] = r ? mmul(df, c16[((16 / R) << l) * r], m) : df;
                 }
-            if (l + 1 < LGR) b = mmul(b, b, m);
-        }
     }
-    // Radix-R DIT over the levels h, 2h, .., Rh/2 of the group x[k] = entry i0 + k h (j = i0 mod h),
-    // with c = w_(Rh)^-j: level l pairs (k, k + e), e = 2^l, and multiplies the second entry by
-    // w_(2h 2^l)^-(j + (k mod e) h) = c^(R / 2^(l+1)) w_16^-((8 >> l) (k mod e)).
+    // Constant DIT network: level l pairs (k, k + e), e = 2^l, the second entry times
+    // w_16^-((8 >> l) r) for r = k mod e (none for r = 0).
     template <int R>
-    __device__ __forceinline__ void dit(u64 (&x)[16], u64 c) const {
-        constexpr int LGR = R == 16 ? 4 : R == 8 ? 3 : R == 4 ? 2 : 1;
-        u64 pw[LGR];  // pw[l] = c^(R >> (l + 1))
-        pw[LGR - 1] = c;
+    __device__ __forceinline__ void dit_c(u64 (&x)[16]) const {
 #pragma unroll
-        for (int l = LGR - 2; l >= 0; --l) pw[l] = mmul(pw[l + 1], pw[l + 1], m);
-#pragma unroll
-        for (int l = 0; l < LGR; ++l) {
-            const int e = 1 << l;
-            u64 tw[8];
-#pragma unroll
-            for (int r = 0; r < e; ++r) tw[r] = r == 0 ? pw[l] : mmul(pw[l], c16i[(8 >> l) * r], m);
+        for (int l = 0, e = 1; e < R; ++l, e <<= 1)
 #pragma unroll
             for (int k0 = 0; k0 < R; k0 += 2 * e)
 #pragma unroll
                 for (int r = 0; r < e; ++r) {
-                    const u64 u = x[k0 + r], o = mmul(x[k0 + r + e], tw[r], m);
+                    const u64 u = x[k0 + r], o = r ? mmul(x[k0 + r + e], c16i[(8 >> l) * r], m) : x[k0 + r + e];
                     x[k0 + r] = madd(u, o, m);
                     x[k0 + r + e] = msub(u, o, m);
                 }
-        }
     }
-    // Forward passes P .. NF-2 (radix 16, h = L/2 / 16^P), one phase each.
+    static constexpr int rev4(int k) { return ((k & 1) << 3) | ((k & 2) << 1) | ((k & 4) >> 1) | ((k & 8) >> 3); }
+    // Entry k times b^rev4(k), k = 1..15: b^2, b^4, b^8 by squaring, the other powers by one
+    // product each (14 products); each power is applied as soon as it is made (the power e goes to
+    // entry rev4(e)), so at most eight are live.
+    __device__ __forceinline__ void twiddle_chain(u64 (&x)[16], u64 b) const {
+        u64 pw[8];
+        pw[1] = b;
+        pw[2] = mmul(b, b, m);
+        pw[3] = mmul(pw[2], b, m);
+        pw[4] = mmul(pw[2], pw[2], m);
+#pragma unroll
+        for (int e = 5; e < 8; ++e) pw[e] = mmul(pw[4], pw[e - 4], m);
+#pragma unroll
+        for (int e = 1; e < 8; ++e) x[rev4(e)] = mmul(x[rev4(e)], pw[e], m);
+        const u64 p8 = mmul(pw[4], pw[4], m);
+        x[rev4(8)] = mmul(x[rev4(8)], p8, m);
+#pragma unroll
+        for (int e = 9; e < 16; ++e) x[rev4(e)] = mmul(x[rev4(e)], mmul(p8, pw[e - 8], m), m);
+    }
+    // Entry k times tab[step rev4(k)], k = 1..15 (w_L^(+-16 step rev4(k)) from W or Wi).
+    __device__ __forceinline__ void twiddle_tab(u64 (&x)[16], const u64* tab, int step) const {
+#pragma unroll
+        for (int k = 1; k < 16; ++k) x[k] = mmul(x[k], tab[step * rev4(k)], m);
+    }
+    // w_L^j and w_L^-j for j < L/16: the roots of order L for j mod 16 times W[j/16]
+    __device__ __forceinline__ u64 wpow(int j) const { return mmul(wsm[j & 15], W[j >> 4], m); }
+    __device__ __forceinline__ u64 wpowi(int j) const { return mmul(wsmi[j & 15], Wi[j >> 4], m); }
+    // Forward passes P .. NF-2 (radix 16, h = L/2 / 16^P, twiddles w_L^(16^P j rev4(k)) from W), one
+    // phase each.
     template <int P>
     __device__ __forceinline__ void fwd_mid() {
         if constexpr (P <= NF - 2) {
-            constexpr int H = (L / 2) >> (4 * P), S = 2 * H / 16;
+            constexpr int H = (L / 2) >> (4 * P), S = 2 * H / 16, F = 1 << (4 * (P - 1));
+#pragma unroll 1
             for (int k = t; k < G1; k += T) {
                 const int j = k & (S - 1), i0 = (k - j) * 16 + j;  // block (k / S) of 2H = 16 S entries
                 u64 x[16];
                 ld<16>(x, i0, S);
-                dif<16>(x, bw[j * (L / (2 * H))]);
+                dif_c<16>(x);
+                twiddle_tab(x, W, F * j);
                 st<16>(x, i0, S);
             }
             sync();
             fwd_mid<P + 1>();
         }
     }
-    // Inverse passes P .. NF-2 (radix 16; h = RP 16^(P-1) after a partial first pass, else 16^P).
+    // Inverse passes P .. NF-2 (radix 16; h = RP 16^(P-1) after a partial first pass, else 16^P;
+    // twiddles w_L^-((L / 16h) j rev4(k)) from Wi), one phase each.
     template <int P>
     __device__ __forceinline__ void inv_mid() {
         if constexpr (P <= NF - 2) {
-            constexpr int H = (LG % 4 ? PP::RP : 16) << (4 * (P - 1));
+            constexpr int H = (LG % 4 ? PP::RP : 16) << (4 * (P - 1)), F = L / (256 * H);
+#pragma unroll 1
             for (int k = t; k < G1; k += T) {
                 const int j = k & (H - 1), i0 = (k - j) * 16 + j;  // block (k / H) of 16 H entries
                 u64 x[16];
                 ld<16>(x, i0, H);
-                dit<16>(x, bwi[j * (L / (16 * H))]);
+                twiddle_tab(x, Wi, F * j);
+                dit_c<16>(x);
                 st<16>(x, i0, H);
             }
             sync();
@@ -282,24 +306,30 @@ struct Work62 {
         for (int k = t; k < L / RM; k += T) {
             u64 y[16];
             ld<RM>(y, k * RM, 1);
-            dif<RM>(y, m.one);
+            dif_c<RM>(y);  // j = 0: no twiddles beyond the constants
 #pragma unroll
             for (int i = 0; i < RM; ++i) y[i] = mmul(y[i], a.hat[k * RM + i], m);
-            dit<RM>(y, m.one);
+            dit_c<RM>(y);
             st<RM>(y, k * RM, 1);
         }
         sync();
         inv_mid<1>();
     }
-    // The first forward pass of group k (entries k + i G1) from registers into LDS.
+    // The first forward pass (h = L/2, j = k) of group k (entries k + i G1) from registers into the buffer.
     __device__ __forceinline__ void first_pass(u64 (&x)[16], int k) {
-        if constexpr (NF >= 2) dif<16>(x, bw[k]);
+        if constexpr (NF >= 2) {
+            dif_c<16>(x);
+            twiddle_chain(x, wpow(k));
+        }
         st<16>(x, k, G1);
     }
-    // The last inverse pass of group k: x[i] = result entry k + i G1.
+    // The last inverse pass (h = L/16, j = k) of group k: x[i] = result entry k + i G1.
     __device__ __forceinline__ void last_pass(u64 (&x)[16], int k) {
         ld<16>(x, k, G1);
-        if constexpr (NF >= 2) dit<16>(x, bwi[k]);
+        if constexpr (NF >= 2) {
+            twiddle_chain(x, wpowi(k));
+            dit_c<16>(x);
+        }
     }
     // Entry (r, c) of a table, row block r/16 the one whose segment slots are in segrow: its tile
     // when column block c/16 has been flushed (it ends before row block r/16), else the staging
@@ -426,8 +456,9 @@ struct Work62 {
             a.qtot[2 * j] = 0;
             a.qtot[2 * j + 1] = g_of(q[d.q_off + j].M);
         }
-        for (int i = t; i < G1; i += T) bw[i] = a.tw[L / 2 + i], bwi[i] = a.twi[L / 2 + i];  // w^i = tw[L/2 + i]
-        if (t < 8) c16[t] = a.tw[8 + t], c16i[t] = a.twi[8 + t];                          // w_16^r = tw[8 + r]
+        for (int x = t; x < G1; x += T) W[x] = wexp(a.tw, 16 * x), Wi[x] = wexp(a.twi, 16 * x);
+        if (t < 16) wsm[t] = wexp(a.tw, t), wsmi[t] = wexp(a.twi, t);
+        if (t < 8) c16[t] = a.tw[8 + t], c16i[t] = a.twi[8 + t];  // w_16^r = tw[8 + r]
         for (int i = t; i < L; i += T) buf[i] = (i >= 1 && i < nm) ? a.inv[i] : 0;
         sync();
         fwd();
@@ -546,12 +577,12 @@ __global__ void __launch_bounds__(P62<LG>::T)
     using PP = P62<LG>;
     __shared__ u64 lbuf[PP::HBM ? 1 : PP::L];
     __shared__ u64 glo[64], ghi[PP::HI], red[PP::NW * kQG];
-    __shared__ u64 bw[PP::G1], bwi[PP::G1], c16[8], c16i[8];
+    __shared__ u64 W[PP::G1], Wi[PP::G1], wsm[16], wsmi[16], c16[8], c16i[8];
     __shared__ int segrow[PP::NSEG];
     const Task62& d = tasks[blockIdx.x];
     const Arena62 ar = arena62(mem, d);
     u64* buf = PP::HBM ? ar.tbuf : lbuf;
-    Work62<LG> w{d, d.m, ar, qs, buf, glo, ghi, red, bw, bwi, c16, c16i, slots, segrow, (int)threadIdx.x};
+    Work62<LG> w{d, d.m, ar, qs, buf, glo, ghi, red, W, Wi, wsm, wsmi, c16, c16i, slots, segrow, (int)threadIdx.x};
     w.run();
     for (int j = threadIdx.x; j < d.q_cnt; j += PP::T) {
         const QueryDesc& qq = qs[d.q_off + j];

@@ -1,10 +1,12 @@
-"""The register passes of the 62-bit kernel (cuckoo-failure_amd/csrc/cf_p62.hip, Work62::dif / dit
-and the pass lists fwd_mid / inv_mid / conv_mid), restated over Python integers: radix-16 DIF passes
-from the top and one partial pass at the bottom, the inverse DIT passes in the opposite order, each
-group's twiddles made from one base w^e (e < L/16) and the roots of order 16. They must give the
-radix-2 transforms word for word (the prologue's fwd() and the textbook DIT inverse), and the
-inverse of the forward must be L times the input. A wrong group index, twiddle exponent or pass
-order fails here for some L."""
+"""The register passes of the 62-bit kernel (cuckoo-failure_amd/csrc/cf_p62.hip, Work62::dif_c /
+dit_c, twiddle_chain / twiddle_tab and the pass lists first_pass / fwd_mid / conv_mid / inv_mid /
+last_pass), restated over Python integers: radix-16 DIF passes from the top and one partial pass at
+the bottom, the inverse DIT passes in the opposite order; each pass a constant network of roots of
+order 16 and one twiddle per entry, from the LDS tables W[x] = w^(16x), Wi[x] = w^-(16x) or, for
+the outermost passes, a chain of powers of w^(+-j). They must give the radix-2 transforms word for
+word (the prologue's fwd() and the textbook DIT inverse), and the inverse of the forward must be L
+times the input. A wrong group index, table index, twiddle exponent or pass order fails here for
+some L."""
 import random
 
 import pytest
@@ -82,46 +84,82 @@ def _passes(lg):
     return fwd, inv
 
 
-def _dif_pass(x, R, h, tw, p):
+def _wexp(tab, e, p):
+    # w^e from tw (or w^-e from twi), e < L: tab[L/2 + e] below L/2, and w^(L/2) = -1 (Work62::wexp)
+    L = len(tab)
+    return tab[L // 2 + e] if e < L // 2 else (p - tab[e]) % p
+
+
+def _rev(k, bits):
+    return int(format(k, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def _dif_c(g, R, tw, p):
+    # the constant DIF network of R entries (Work62::dif_c): level l pairs (k, k + e), e = R/2^(l+1),
+    # the difference times w_16^((16/R) 2^l r), r = k mod e
+    l, e = 0, R // 2
+    while e:
+        for k0 in range(0, R, 2 * e):
+            for r in range(e):
+                u, v = g[k0 + r], g[k0 + r + e]
+                g[k0 + r], g[k0 + r + e] = (u + v) % p, (u - v) * (tw[8 + ((16 // R) << l) * r] if r else 1) % p
+        l, e = l + 1, e // 2
+
+
+def _dit_c(g, R, twi, p):
+    # the constant DIT network (Work62::dit_c): level l pairs (k, k + 2^l), the second entry times
+    # w_16^-((8 >> l) r), r = k mod 2^l
+    l, e = 0, 1
+    while e < R:
+        for k0 in range(0, R, 2 * e):
+            for r in range(e):
+                u, o = g[k0 + r], g[k0 + r + e] * (twi[8 + (8 >> l) * r] if r else 1) % p
+                g[k0 + r], g[k0 + r + e] = (u + o) % p, (u - o) % p
+        l, e = l + 1, e * 2
+
+
+def _chain(g, b, p):
+    # entry k times b^rev4(k) (Work62::twiddle_chain)
+    for k in range(1, 16):
+        g[k] = g[k] * pow(b, _rev(k, 4), p) % p
+
+
+def _dif_pass(x, R, h, tw, p, P):
+    # forward pass P (Work62::first_pass for P = 0, fwd_mid, and the shared middle phase for the
+    # partial pass): constant network, then entry k times w_2h^(j rev(k))
     L, s = len(x), 2 * h // R
+    W = [_wexp(tw, 16 * i, p) for i in range(max(1, L // 16))]
     for k in range(L // R):
         j = k % s
-        i0 = (k - j) * 16 + j if R == 16 else k * R  # the kernel's group index (partial pass: j = 0)
-        b = tw[L // 2 + j * (L // (2 * h))]  # the LDS base table w^e, e < L/16
-        assert j * (L // (2 * h)) < max(1, L // 16)
+        i0 = (k - j) * 16 + j if R == 16 else k * R
         g = [x[i0 + m * s] for m in range(R)]
-        l, e = 0, R // 2
-        while e:
-            tws = [b if r == 0 else b * tw[8 + ((16 // R) << l) * r] % p for r in range(e)]
-            for k0 in range(0, R, 2 * e):
-                for r in range(e):
-                    u, v = g[k0 + r], g[k0 + r + e]
-                    g[k0 + r], g[k0 + r + e] = (u + v) % p, (u - v) * tws[r] % p
-            b, l, e = b * b % p, l + 1, e // 2
+        _dif_c(g, R, tw, p)
+        if R == 16 and j:
+            if P == 0:  # base w^j = w^(j mod 16) W[j / 16]
+                _chain(g, _wexp(tw, j & 15, p) * W[j >> 4] % p, p)
+            else:  # W[16^(P-1) j rev4(k)]
+                for m in range(1, 16):
+                    g[m] = g[m] * W[(1 << (4 * (P - 1))) * j * _rev(m, 4)] % p
         for m in range(R):
             x[i0 + m * s] = g[m]
 
 
 def _dit_pass(x, R, h, twi, p):
+    # inverse pass (Work62::inv_mid, last_pass, the shared middle phase): entry k times
+    # w_(Rh)^-(j rev(k)), then the constant network
     L = len(x)
+    Wi = [_wexp(twi, 16 * i, p) for i in range(max(1, L // 16))]
     for k in range(L // R):
         j = k % h
         i0 = (k - j) * 16 + j if R == 16 else k * R
-        c = twi[L // 2 + j * (L // (R * h))]
-        assert j * (L // (R * h)) < max(1, L // 16)
         g = [x[i0 + m * h] for m in range(R)]
-        lgr = R.bit_length() - 1
-        pw = [0] * lgr
-        pw[lgr - 1] = c
-        for l in range(lgr - 2, -1, -1):
-            pw[l] = pw[l + 1] * pw[l + 1] % p
-        for l in range(lgr):
-            e = 1 << l
-            tws = [pw[l] if r == 0 else pw[l] * twi[8 + (8 >> l) * r] % p for r in range(e)]
-            for k0 in range(0, R, 2 * e):
-                for r in range(e):
-                    u, o = g[k0 + r], g[k0 + r + e] * tws[r] % p
-                    g[k0 + r], g[k0 + r + e] = (u + o) % p, (u - o) % p
+        if R == 16 and j:
+            if h == L // 16:  # the last pass: base w^-j
+                _chain(g, _wexp(twi, j & 15, p) * Wi[j >> 4] % p, p)
+            else:  # Wi[(L / 256h) j rev4(k)]
+                for m in range(1, 16):
+                    g[m] = g[m] * Wi[(L // (256 * h)) * j * _rev(m, 4)] % p
+        _dit_c(g, R, twi, p)
         for m in range(R):
             x[i0 + m * h] = g[m]
 
@@ -136,8 +174,8 @@ def test_register_passes_equal_the_radix_2_transforms(lg):
     want = _dif2(x, tw, p)
     fwd, inv = _passes(lg)
     y = x[:]
-    for R, h in fwd:
-        _dif_pass(y, R, h, tw, p)
+    for P, (R, h) in enumerate(fwd):
+        _dif_pass(y, R, h, tw, p, P)
     assert y == want
     back = _dit2(want, twi, p)
     for R, h in inv:
