@@ -15,7 +15,8 @@ vector peak of 78.6 TFLOP/s. `cpu_baseline` times the reference's own host build
 its unmodified sources into oracle/_ref) on this machine's cores, on a bounded sample of the
 same workload. `traffic` is the HBM bytes of one launch from rocprofv3 FETCH_SIZE / WRITE_SIZE
 passes scaled by the counters' readings for a plain copy of known size; it is read from the
-committed profile `profiles/k_tasks14_n16384_traffic.json` (null if that file is absent).
+committed profile `profiles/k_tasks14_n16384_traffic.json` (null if that file is absent);
+`hbm_bytes_per_s` and `hbm_frac` are that traffic over the measured launch time, against 8 TB/s.
 """
 import argparse
 import json
@@ -31,6 +32,7 @@ sys.path.insert(0, os.path.join(REPO, "cuckoo-failure_amd"))
 
 N, M, D, NPRIMES = 16384, 32768, 3, 512
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector, dense
+PEAK_HBM_BYTES_PER_S = 8e12  # MI355X HBM3E
 FLOP_PER_MODMUL = 12         # 6 FP64 VALU instructions, counted as FMA-equivalents (2 flops each)
 
 
@@ -426,7 +428,7 @@ def main():
     ap.add_argument("--queries", type=int, default=100000)
     ap.add_argument("--rate", type=float, default=1500.0)
     ap.add_argument("--seq-queries", type=int, default=1000)
-    ap.add_argument("--serve-queries", type=int, default=20000,
+    ap.add_argument("--serve-queries", type=int, default=100000,
                     help="fast: requests of the serving stream replayed after the headline (0: no serving figures)")
     ap.add_argument("--slots", type=int, default=2)
     ap.add_argument("--max-batch", type=int, default=32)
@@ -479,6 +481,7 @@ def main():
     achieved = mm_per_launch * FLOP_PER_MODMUL / (per_launch_ms * 1e-3) / 1e12
     base = None if a.cpu_baseline == "none" else cpu_baseline(a.cpu_baseline)
     mpeak = measured_modmul_peak()
+    traffic = profiled_traffic()
     out = {
         "metric": "prime-residues/s at n=16384,d=3 (bit-exact); nbad queries/s + p99 latency",
         "value": round(value, 3),
@@ -497,7 +500,9 @@ def main():
                    "primes": a.primes, "queries_per_prime": 1, "device": dev},
         "roofline": {"bound": "valu", "kernel": "k_tasks<14>", "achieved": round(achieved, 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-                     "traffic": profiled_traffic(), "launch_ms": round(per_launch_ms, 2),
+                     "traffic": traffic, "launch_ms": round(per_launch_ms, 2),
+                     "hbm_bytes_per_s": round(traffic / (per_launch_ms * 1e-3)) if traffic else None,
+                     "hbm_frac": round(traffic / (per_launch_ms * 1e-3) / PEAK_HBM_BYTES_PER_S, 4) if traffic else None,
                      "modmul_per_launch": mm_per_launch,
                      "measured_modmul_peak_per_s": mpeak["fp64_exact_modmul_per_s"] if mpeak else None,
                      "frac_of_measured_modmul_peak": round(mm_per_launch / (per_launch_ms * 1e-3) /
@@ -512,7 +517,7 @@ def main():
         out["autotune"] = tune
         if tune["chosen"]:  # the profiled traffic is the built-in variant's
             out["roofline"]["kernel"] = f"k_tasks<14> variant {tune['chosen']}"
-            out["roofline"]["traffic"] = None
+            out["roofline"]["traffic"] = out["roofline"]["hbm_bytes_per_s"] = out["roofline"]["hbm_frac"] = None
     print(json.dumps(out))
 
 
