@@ -139,13 +139,22 @@ struct Work62 {
         if (v < d.dd) return 0;
         return mmul(mmul(a.fact[v], a.ifact[d.dd], m), a.ifact[v - d.dd], m);
     }
-    __device__ __forceinline__ u64 gpow(int e) const { return mmul(ghi[e >> 6], glo[e & 63], m); }
-    // g^i (i < 64) and g^(64 j) (64 j <= nm + 64); the caller's next barrier publishes them
-    __device__ __forceinline__ void power_tables(u64 g) {
+    // g^e v! and g^e / v! (e <= nm): the low table carries the step's factor
+    __device__ __forceinline__ u64 gpow_f(int e) const { return mmul(ghi[e >> 6], glo[64 + (e & 63)], m); }
+    __device__ __forceinline__ u64 gpow_i(int e) const { return mmul(ghi[e >> 6], glo[128 + (e & 63)], m); }
+    // g^i, g^i v!, g^i / v! (i < 64) and g^(64 j) (64 j <= nm + 64) for g = g(v); the caller's next
+    // barrier publishes them
+    __device__ __forceinline__ void power_tables(u64 g, int v) {
         const int nhi = (d.nm >> 6) + 2;
         for (int i = t; i < 64 + nhi; i += T) {
-            if (i < 64) glo[i] = mpow(g, (u64)i, m);
-            else ghi[i - 64] = mpow(mpow(g, 64, m), (u64)(i - 64), m);
+            if (i < 64) {
+                const u64 gi = mpow(g, (u64)i, m);
+                glo[i] = gi;
+                glo[64 + i] = mmul(gi, a.fact[v], m);
+                glo[128 + i] = mmul(gi, a.ifact[v], m);
+            } else {
+                ghi[i - 64] = mpow(mpow(g, 64, m), (u64)(i - 64), m);
+            }
         }
     }
     // Forward DIF, natural order in, bit-reversed out, by radix-2 levels h = L/2, .., 1 (pairs (i, i + h)
@@ -473,7 +482,7 @@ struct Work62 {
             if (v > v0) {
                 // stage-1 input of row v, the query sums, the first forward pass; g's power tables
                 const u64 iv = a.inv[v];
-                power_tables(g_of(v));
+                power_tables(g_of(v), v);
                 if (nact > kQG) {  // many queries: their sums from the input in LDS first
                     for (int k = t; k < G1; k += T) {
                         u64 x[16];
@@ -514,14 +523,13 @@ struct Work62 {
                 if (nact > 0 && nact <= kQG) query_finish(v, 0, nact);
             } else {
                 for (int e = v + t; e < nm; e += T) scol[e] = 0;
-                if (v == v0) power_tables(g_of(v));
+                if (v == v0) power_tables(g_of(v), v);
             }
             sync();
             if (v >= v0) {
-                const u64 iv = a.ifact[v], s1 = mmul(gpow(v), iv, m);
+                const u64 s1 = gpow_i(v);  // g^v / v!
                 if (v > v0) {
                     // stage-2 input b[i] = S2[v][i] v! g^(v-i), 1 <= i <= v; the first forward pass
-                    const u64 fv = a.fact[v];
                     for (int k = t; k < G1; k += T) {
                         u64 x[16];
 #pragma unroll
@@ -530,7 +538,7 @@ struct Work62 {
                             const bool ok = e >= 1 && e <= v;
                             const int ce = ok ? e : 1;
                             const u64 y = s_at(v, ce);
-                            x[i] = ok ? mmul(mmul(y, fv, m), gpow(v - ce), m) : 0;
+                            x[i] = ok ? mmul(y, gpow_f(v - ce), m) : 0;
                         }
                         first_pass(x, k);
                     }
@@ -545,14 +553,14 @@ struct Work62 {
                             const int e = k + i * G1, p = e == 0 ? L : e;
                             if (p > v && p <= nm) {
                                 const u64 pc = mmul(a.mont[p], x[i], m);
-                                const u64 f = mmul(mmul(gpow(p - v), iv, m), a.ifact2[p - v - 1], m);
+                                const u64 f = mmul(gpow_i(p - v), a.ifact2[p - v - 1], m);
                                 vcol[p] = mmul(msub(s1, pc, m), f, m);
                             }
                         }
                     }
                 } else {
                     for (int p = v + 1 + t; p <= nm; p += T) {
-                        const u64 f = mmul(mmul(gpow(p - v), iv, m), a.ifact2[p - v - 1], m);
+                        const u64 f = mmul(gpow_i(p - v), a.ifact2[p - v - 1], m);
                         vcol[p] = mmul(s1, f, m);
                     }
                 }
@@ -576,7 +584,7 @@ __global__ void __launch_bounds__(P62<LG>::T)
           u64* __restrict__ mem, u64* __restrict__ res) {
     using PP = P62<LG>;
     __shared__ u64 lbuf[PP::HBM ? 1 : PP::L];
-    __shared__ u64 glo[64], ghi[PP::HI], red[PP::NW * kQG];
+    __shared__ u64 glo[3 * 64], ghi[PP::HI], red[PP::NW * kQG];
     __shared__ u64 W[PP::G1], Wi[PP::G1], wsm[16], wsmi[16], c16[8], c16i[8];
     __shared__ int segrow[PP::NSEG];
     const Task62& d = tasks[blockIdx.x];

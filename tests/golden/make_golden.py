@@ -18,6 +18,7 @@ the programs' outputs are kept, as data.
     python tests/golden/make_golden.py --only big        # n = 65536: primes 1..3, and 126 queries on 0..1
     python tests/golden/make_golden.py --only fast62     # the reference's -DFP_RB=64 build (62-bit primes)
     python tests/golden/make_golden.py --only fast62wide # the same build on tables beyond 16384 rows
+    python tests/golden/make_golden.py --only fast62big  # and on one table of 65537 rows (2^17)
     python tests/golden/make_golden.py --only big17      # n = 70000 (transforms of 2^17), prime 0 (~39 GB, ~4 min)
 """
 import json
@@ -477,6 +478,15 @@ def fast62_wide_cases():
         return list(ex.map(one, cases))
 
 
+def _qfile(qs):
+    """A query file in a temporary directory that lives until the process ends."""
+    td = tempfile.mkdtemp()
+    qf = os.path.join(td, "q.txt")
+    with open(qf, "w") as f:
+        f.write("".join(f"{n} {M}\n" for n, M in qs))
+    return qf
+
+
 def _write(name, data):
     """Writes one fixture file after its data is complete (an interrupted run leaves the old file)."""
     tmp = os.path.join(HERE, name + ".tmp")
@@ -517,6 +527,11 @@ def main():
         return
     if only == "fast62wide":
         _write("fast62_wide.json", fast62_wide_cases())
+        return
+    if only == "fast62big":  # one table of 65537 rows: a transform of 2^17
+        _write("fast62_65537.json", {"d": 3, "queries": [[65537, 65540], [40000, 80000]], "first": 2, "count": 1,
+                                     "stdout": run([os.path.join(REF, "cuckoo_fast62"), "3", _qfile([(65537, 65540), (40000, 80000)]),
+                                                    "2", "1"])[1]})
         return
     if only == "separate":
         _write("fail_separate_cases.json", fail_separate_cases())

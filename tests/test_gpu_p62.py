@@ -155,6 +155,24 @@ def test_62_bit_tables_beyond_16384_rows(eng, g):
 
 
 @pytest.mark.gpu
+@pytest.mark.skipif(os.environ.get("CF_SLOW_TESTS") != "1",
+                    reason="a table of 65537 rows takes about 126 s on one CU of an MI355X; run with CF_SLOW_TESTS=1")
+def test_62_bit_table_of_65537_rows(eng):
+    # the longest transform, 2^17 (its twiddle tables fill the LDS): the reference's -DFP_RB=64 lines
+    # for two queries on prime 2
+    try:
+        rec = load("fast62_65537.json")
+    except OSError:
+        pytest.skip("fixture not generated")
+    lines = residue_lines(rec["stdout"])
+    qs = [tuple(q) for q in rec["queries"]]
+    want = {(n, M): r for _, _, n, M, r in lines}
+    print(f"d=3 queries={qs} prime {rec['first']}", flush=True)
+    got = eng.run(qs, [lines[0][1]], 3)
+    assert got[0] == [want[q] for q in qs]
+
+
+@pytest.mark.gpu
 def test_62_bit_beyond_131072_rows_is_refused(eng):
     with pytest.raises(cf.CfError) as ei:
         eng.run([(131073, 131074)], P62[:1], 3)
