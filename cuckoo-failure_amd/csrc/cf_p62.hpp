@@ -9,7 +9,7 @@
 
 namespace cf {
 
-constexpr int kP62Threads = 1024;  // threads of the longest transform's workgroup (L = 2^14)
+constexpr int kP62Threads = 1024;  // bound on the kernel's workgroup size (it uses at most 512 threads)
 constexpr int kP62MaxLog2L = 17;   // tables of up to 131072 rows (transforms in LDS up to 2^14, in HBM beyond)
 
 // Montgomery arithmetic modulo an odd p < 2^63 with R = 2^64.
