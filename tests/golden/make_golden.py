@@ -455,14 +455,16 @@ def fast62_cases():
 
 def fast62_wide_cases():
     """The reference's -DFP_RB=64 build on tables of more than 16384 rows (transforms of 2^15 and
-    2^16), one prime per process: several queries per table, d = 2, 3, 4."""
+    2^16), one prime per process: several queries per table, d = 1 to 5."""
     fast62 = os.path.join(REF, "cuckoo_fast62")
     cases = [(3, [(16385, 16386), (20000, 40000), (17000, 20000)], 0, 1),
              (3, [(16385, 16386), (20000, 40000), (17000, 20000)], 1, 1),
              (2, [(16500, 17000)], 2, 1),
              (4, [(16400, 16401)], 3, 1),
              (3, [(32768, 65536)], 0, 1),
-             (3, [(40000, 80000)], 1, 1)]
+             (3, [(40000, 80000)], 1, 1),
+             (1, [(16390, 16391), (16400, 20000)], 0, 1),
+             (5, [(16400, 16500)], 3, 1)]
 
     def one(c):
         d, qs, first, count = c

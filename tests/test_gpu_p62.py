@@ -138,10 +138,10 @@ def _wide_groups():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("g", range(5))
+@pytest.mark.parametrize("g", range(7))
 def test_62_bit_tables_beyond_16384_rows(eng, g):
     # transforms of 2^15 and 2^16 with the buffer in HBM: every line of the reference's
-    # -DFP_RB=64 build for tables of 16385 to 40000 rows
+    # -DFP_RB=64 build for tables of 16385 to 40000 rows, d = 1 to 5
     groups = _wide_groups()
     if g >= len(groups):
         pytest.skip("fixture not generated")
@@ -152,6 +152,24 @@ def test_62_bit_tables_beyond_16384_rows(eng, g):
     got = eng.run(list(qs), primes, d)
     for i, p, n, M, r in lines:
         assert got[idx.index(i)][list(qs).index((n, M))] == r, (d, i, n, M)
+
+
+@pytest.mark.gpu
+def test_cuckoo_fast62_beyond_16384_rows_prints_the_reference_lines():
+    # the driver end to end on a table of 16400 rows (d = 4, prime 3): the reference's output bytes
+    try:
+        recs = load("fast62_wide.json")
+    except OSError:
+        pytest.skip("fixture not generated")
+    rec = next(r for r in recs if r["d"] == 4)
+    with tempfile.TemporaryDirectory() as td:
+        qf = os.path.join(td, "q.txt")
+        with open(qf, "w") as f:
+            f.write("".join(f"{n} {M}\n" for n, M in rec["queries"]))
+        r = subprocess.run([CLI_FAST62, str(rec["d"]), qf, str(rec["first"]), str(rec["count"])], capture_output=True,
+                           text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == rec["stdout"]
 
 
 @pytest.mark.gpu
