@@ -287,7 +287,7 @@ def _lines_by_prime(stdout):
 
 # Successful reference runs are cached by their arguments (the long fixtures take tens of minutes
 # of CPU, so an interrupted generation resumes where it stopped). Set CF_GOLDEN_CACHE="" to disable.
-CACHE = os.environ.get("CF_GOLDEN_CACHE", os.path.join(REPO, "_work", "golden_cache"))
+CACHE = os.environ.get("CF_GOLDEN_CACHE", os.path.join(tempfile.gettempdir(), "cf_golden_cache"))
 
 
 def cached_run(d, queries, first, count):
