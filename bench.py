@@ -51,10 +51,16 @@ def table_work(d, ns):
     return work_modmul(nm, d, 0) + 1.5 * sum(n * n for n in ns)
 
 
-def cpu_baseline(kind, procs=None):
+REF_BUILDS = {"ifma": ("cuckoo_fast_ifma", "AVX-512 IFMA host build"), "scalar": ("cuckoo_fast_cpu", "scalar host build"),
+              "fast62": ("cuckoo_fast62", "-DFP_RB=64 host build (62-bit primes, scalar)")}
+
+
+def cpu_baseline(kind, procs=None, residues=None):
     """The reference host build on `procs` cores: one process per prime (the reference's own
-    scale-out by disjoint prime ranges), wall time from first start to last exit."""
-    exe = os.path.join(REPO, "oracle", "_ref", "cuckoo_fast_ifma" if kind == "ifma" else "cuckoo_fast_cpu")
+    scale-out by disjoint prime ranges), wall time from first start to last exit. When `residues`
+    is a dict, the printed residues are put in it by prime index."""
+    name, what = REF_BUILDS[kind]
+    exe = os.path.join(REPO, "oracle", "_ref", name)
     if not os.path.exists(exe):
         return None
     if kind == "ifma" and "avx512ifma" not in open("/proc/cpuinfo").read():
@@ -72,9 +78,13 @@ def cpu_baseline(kind, procs=None):
         wall = time.perf_counter() - t0
     if any(p.returncode for p in ps) or sum(len(o.splitlines()) for o in outs) != procs:
         return None
+    if residues is not None:
+        for o in outs:
+            t = o.split()
+            residues[int(t[0])] = int(t[4])
     return {"value": procs / wall, "unit": "prime-residues/s", "cores": procs, "kind": "reference",
             "sample": f"{procs} primes (i=0..{procs - 1}) of n={N}, M={M}, d={D}, one process per prime, "
-                      f"{'AVX-512 IFMA' if kind == 'ifma' else 'scalar'} host build, wall {wall:.1f} s"}
+                      f"{what}, wall {wall:.1f} s"}
 
 
 def reference_residues(name):
@@ -389,9 +399,16 @@ def bench_fast62(a):
                         ref[int(t[0])] = int(t[4])
     except (OSError, ValueError):
         pass
+    base = None
+    if a.cpu_baseline != "none":  # the reference's 62-bit build on the host cores, its residues checked too
+        timed = {}
+        base = cpu_baseline("fast62", residues=timed)
+        ref.update(timed)
     pinned = [i for i in ref if i < count]
     assert all(got[i][0] == ref[i] for i in pinned), "62-bit residues differ from the reference's"
     mm = work_modmul(N, D) * count
+    mpeak = measured_modmul_peak()
+    mont = mpeak.get("mont64_modmul_per_s") if mpeak else None
     print(json.dumps({
         "metric": "prime-residues/s at n=16384,d=3, 62-bit primes (bit-exact)",
         "value": round(count / wall, 3), "unit": "prime-residues/s", "n_gpus": 1, "higher_is_better": True,
@@ -401,6 +418,10 @@ def bench_fast62(a):
                    "device": dev, "launches": st.launches, "kernel_ms": round(st.kernel_ms, 1)},
         "equivalent_50_bit_residues_per_s": round(count / wall * 61 / 49, 3),
         "modmul_per_s": round(mm / (st.kernel_ms * 1e-3), 1),
+        "roofline": {"bound": "valu", "kernel": "k_p62<14>", "achieved": round(mm / (st.kernel_ms * 1e-3), 1),
+                     "peak": mont, "unit": "Montgomery products/s (measured peak, bin/cf_modmul_rate)",
+                     "frac": round(mm / (st.kernel_ms * 1e-3) / mont, 4) if mont else None, "traffic": None},
+        "cpu_baseline": base,
         "reference_check": f"residues of primes {sorted(pinned)} equal the reference's 62-bit build's",
     }))
 
