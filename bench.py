@@ -55,6 +55,11 @@ REF_BUILDS = {"ifma": ("cuckoo_fast_ifma", "AVX-512 IFMA host build"), "scalar":
               "fast62": ("cuckoo_fast62", "-DFP_RB=64 host build (62-bit primes, scalar)")}
 
 
+def progress(msg):
+    """One line on stderr as each stage of the run ends (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(kind, procs=None, residues=None):
     """The reference host build on `procs` cores: one process per prime (the reference's own
     scale-out by disjoint prime ranges), wall time from first start to last exit. When `residues`
@@ -122,6 +127,7 @@ def serving_leg(a):
     t0 = time.perf_counter()
     tabs = cf.Tables(eng, 3, nmax, Mmax, cf.prime_range(0, np_))
     build_s = time.perf_counter() - t0
+    progress(f"serving: tables built in {build_s:.2f} s; replaying {k_srv} requests")
     checked = 0
     try:
         with open(os.path.join(REPO, "tests", "golden", "serving_2048.json")) as f:
@@ -137,6 +143,7 @@ def serving_leg(a):
     with cf.Server(tabs, slots=a.slots, max_batch=a.max_batch, crt_threads=a.crt_threads) as srv:
         srv.replay(n[:64], M[:64], t[:64] - t[0])  # warm-up
         _, l2, _, st = srv.replay(n[:k_srv], M[:k_srv], t[:k_srv] - t[0])
+    progress(f"serving: {k_srv} requests replayed; {k_seq} more one per launch")
     with cf.Server(tabs, slots=1, max_batch=1, crt_threads=a.crt_threads) as seq:
         _, l2s, _, sst = seq.replay(n[:k_seq], M[:k_seq], t[:k_seq] - t[0])
     assert l2s == l2[:k_seq], "served and sequential answers differ"
@@ -476,6 +483,7 @@ def main():
     tune = eng.autotune(14, n=N) if a.autotune else None
     for _ in range(a.warmup):
         eng.run([(N, M)], primes, D)
+    progress(f"headline: {a.warmup} warm-up step(s) done")
     kernel_ms, launches, tasks = 0.0, 0, 0
     outs = []
     t0 = time.perf_counter()
@@ -486,6 +494,7 @@ def main():
         launches += st.launches
         tasks += st.tasks
     wall = time.perf_counter() - t0
+    progress(f"headline: {a.steps} timed step(s) in {wall:.2f} s")
     dev = eng.device_name()
     eng.close()
     # every residue of every timed call against the reference host build's lines for the slice
@@ -501,7 +510,9 @@ def main():
     mm_per_launch = work_modmul(N, D) * tasks / launches
     achieved = mm_per_launch * FLOP_PER_MODMUL / (per_launch_ms * 1e-3) / 1e12
     base = None if a.cpu_baseline == "none" else cpu_baseline(a.cpu_baseline)
+    progress(f"cpu baseline: {base['sample'] if base else 'not run'}")
     mpeak = measured_modmul_peak()
+    progress("modmul peak measured")
     traffic = profiled_traffic()
     out = {
         "metric": "prime-residues/s at n=16384,d=3 (bit-exact); nbad queries/s + p99 latency",
