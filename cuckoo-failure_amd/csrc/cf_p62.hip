@@ -191,9 +191,10 @@ struct Work62 {
     __device__ __forceinline__ u64 wexp(const u64* tab, int e) const {
         return e < L / 2 ? tab[L / 2 + e] : msub(0, tab[e], m);
     }
-    // ---- the v-loop's transforms: groups of R entries in registers, one LDS round trip per pass ----
-    // LDS word of transform entry i: bits 0-4 XORed with bits 4-8 (a bijection of [0, L)), so the
-    // strided groups of every pass spread over the banks.
+    // ---- the v-loop's transforms: groups of R entries in registers, one round trip through the
+    //      buffer (LDS, or HBM beyond 2^14) per pass ----
+    // Buffer word of transform entry i: bits 0-4 XORed with bits 4-8 (a bijection of [0, L)), so the
+    // strided groups of every pass spread over the LDS banks.
     __device__ __forceinline__ static int sw(int i) { return i ^ ((i >> 4) & 31); }
     template <int R>
     __device__ __forceinline__ void ld(u64 (&x)[16], int i0, int s) const {
