@@ -48,6 +48,14 @@ __device__ __forceinline__ u64 mmul(u64 a, u64 b, const Mod62& m) {
     const u64 h2 = mul_hi(q, m.p);
     return hi >= h2 ? hi - h2 : hi - h2 + m.p;
 }
+// a v mod p for a constant v < p with q' = floor(v 2^64 / p) (Shoup's product): with q =
+// hi(a q'), a v - q p lies in [0, 2p) and is exact mod 2^64 (2p < 2^64), one conditional
+// subtraction. With a in Montgomery form and v plain, the result is the Montgomery form of the
+// product, the residue mmul gives for v's Montgomery form, in about two thirds of its instructions.
+__device__ __forceinline__ u64 smul(u64 a, u64 v, u64 vq, const Mod62& m) {
+    const u64 r = a * v - mul_hi(a, vq) * m.p;
+    return r >= m.p ? r - m.p : r;
+}
 __device__ __forceinline__ u64 madd(u64 a, u64 b, const Mod62& m) {
     const u64 s = a + b;
     return s >= m.p ? s - m.p : s;
@@ -91,6 +99,7 @@ struct P62 {
 struct Arena62 {
     u64 *fact, *ifact, *inv, *ifact2, *mont;  // F1 each (Montgomery form; mont[i] = i R mod p)
     u64 *tw, *twi;                            // L each: tw[h + j] = w_{2h}^j (and inverses)
+    u64* shc;                                 // 32: w_16^r, their Shoup quotients, w_16^-r, quotients
     u64* hat;                                 // L: spectrum of the kernel times 1/L (bit-reversed order)
     u64 *Vpool, *Spool;                       // tiles of 16 x 16 entries, row-major
     u64 *Vs, *Ss;                             // staging: 2 blocks x 16 columns x NS rows, column-major
@@ -107,7 +116,8 @@ __device__ __forceinline__ Arena62 arena62(u64* mem, const Task62& d) {
     a.mont = a.ifact2 + F1;
     a.tw = a.mont + F1;
     a.twi = a.tw + L;
-    a.hat = a.twi + L;
+    a.shc = a.twi + L;
+    a.hat = a.shc + 32;
     a.Vpool = a.hat + L;
     a.Spool = a.Vpool + tiles;
     a.Vs = a.Spool + tiles;
@@ -127,8 +137,8 @@ struct Work62 {
     const QueryDesc* q;
     u64* buf;
     u64 *glo, *ghi, *red;
-    u64 *W, *Wi, *wsm, *wsmi, *c16, *c16i;  // LDS: w^(16x), w^-(16x) (x < G1), w^b, w^-b (b < 16) for w of
-                                            // order L; w_16^r, w_16^-r (r < 8)
+    u64 *W, *Wi, *wsm, *wsmi, *c16;  // LDS: w^(16x), w^-(16x) (x < G1), w^b, w^-b (b < 16) for w of order
+                                     // L; the Shoup constants of w_16^r and w_16^-r (r < 8), as in shc
     const int* slots;            // segment slots of the plan, Y * NSEG + s
     int* segrow;                 // LDS: the segment slots of the current row block
     int t;
@@ -213,7 +223,7 @@ struct Work62 {
     // times w_(Rh)^-(j rev(k)), then the constant DIT network. Same products as the radix-2 levels,
     // word for word.
     // Constant DIF network: level l pairs (k, k + e), e = R / 2^(l+1), the difference times
-    // w_16^((16/R) 2^l r) for r = k mod e (none for r = 0).
+    // w_16^((16/R) 2^l r) for r = k mod e (none for r = 0), by Shoup's product.
     template <int R>
     __device__ __forceinline__ void dif_c(u64 (&x)[16]) const {
 #pragma unroll
@@ -224,11 +234,12 @@ struct Work62 {
                 for (int r = 0; r < e; ++r) {
                     const u64 u = x[k0 + r], w = x[k0 + r + e], df = msub(u, w, m);
                     x[k0 + r] = madd(u, w, m);
-                    x[k0 + r + e] = r ? mmul(df, c16[((16 / R) << l) * r], m) : df;
+                    const int i = ((16 / R) << l) * r;
+                    x[k0 + r + e] = r ? smul(df, c16[i], c16[8 + i], m) : df;
                 }
     }
     // Constant DIT network: level l pairs (k, k + e), e = 2^l, the second entry times
-    // w_16^-((8 >> l) r) for r = k mod e (none for r = 0).
+    // w_16^-((8 >> l) r) for r = k mod e (none for r = 0), by Shoup's product.
     template <int R>
     __device__ __forceinline__ void dit_c(u64 (&x)[16]) const {
 #pragma unroll
@@ -237,7 +248,8 @@ struct Work62 {
             for (int k0 = 0; k0 < R; k0 += 2 * e)
 #pragma unroll
                 for (int r = 0; r < e; ++r) {
-                    const u64 u = x[k0 + r], o = r ? mmul(x[k0 + r + e], c16i[(8 >> l) * r], m) : x[k0 + r + e];
+                    const int i = (8 >> l) * r;
+                    const u64 u = x[k0 + r], o = r ? smul(x[k0 + r + e], c16[16 + i], c16[24 + i], m) : x[k0 + r + e];
                     x[k0 + r] = madd(u, o, m);
                     x[k0 + r + e] = msub(u, o, m);
                 }
@@ -468,7 +480,7 @@ struct Work62 {
         }
         for (int x = t; x < G1; x += T) W[x] = wexp(a.tw, 16 * x), Wi[x] = wexp(a.twi, 16 * x);
         if (t < 16) wsm[t] = wexp(a.tw, t), wsmi[t] = wexp(a.twi, t);
-        if (t < 8) c16[t] = a.tw[8 + t], c16i[t] = a.twi[8 + t];  // w_16^r = tw[8 + r]
+        if (t < 32) c16[t] = a.shc[t];
         for (int i = t; i < L; i += T) buf[i] = (i >= 1 && i < nm) ? a.inv[i] : 0;
         sync();
         fwd();
@@ -586,12 +598,12 @@ __global__ void __launch_bounds__(P62<LG>::T)
     using PP = P62<LG>;
     __shared__ u64 lbuf[PP::HBM ? 1 : PP::L];
     __shared__ u64 glo[3 * 64], ghi[PP::HI], red[PP::NW * kQG];
-    __shared__ u64 W[PP::G1], Wi[PP::G1], wsm[16], wsmi[16], c16[8], c16i[8];
+    __shared__ u64 W[PP::G1], Wi[PP::G1], wsm[16], wsmi[16], c16[32];
     __shared__ int segrow[PP::NSEG];
     const Task62& d = tasks[blockIdx.x];
     const Arena62 ar = arena62(mem, d);
     u64* buf = PP::HBM ? ar.tbuf : lbuf;
-    Work62<LG> w{d, d.m, ar, qs, buf, glo, ghi, red, W, Wi, wsm, wsmi, c16, c16i, slots, segrow, (int)threadIdx.x};
+    Work62<LG> w{d, d.m, ar, qs, buf, glo, ghi, red, W, Wi, wsm, wsmi, c16, slots, segrow, (int)threadIdx.x};
     w.run();
     for (int j = threadIdx.x; j < d.q_cnt; j += PP::T) {
         const QueryDesc& qq = qs[d.q_off + j];
@@ -609,7 +621,7 @@ hipError_t launch_lg(int ntasks, const Task62* tasks, const QueryDesc* qs, const
 }  // namespace
 
 int64_t arena62_words(int NS, int64_t pool, int F1, int L, int q_cnt) {
-    return 5 * (int64_t)F1 + 3 * (int64_t)L + 512 * pool + 64 * (int64_t)NS + 2 * (int64_t)q_cnt +
+    return 5 * (int64_t)F1 + 3 * (int64_t)L + 32 + 512 * pool + 64 * (int64_t)NS + 2 * (int64_t)q_cnt +
            (L > (1 << 14) ? (int64_t)L : 0);
 }
 

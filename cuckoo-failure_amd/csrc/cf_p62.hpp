@@ -31,7 +31,7 @@ struct Task62 {
     int64_t pool;        // tiles per table
 };
 
-// Words of one task's arena: five factorial-range tables, three of length L, the two tile pools
+// Words of one task's arena: five factorial-range tables, three of length L, 32 Shoup constants, the two tile pools
 // (pool tiles of 256 words each), the staging of both tables (32 columns of NS rows), the query
 // state and the transform's buffer when it does not fit in LDS (L > 2^14).
 int64_t arena62_words(int NS, int64_t pool, int F1, int L, int q_cnt);

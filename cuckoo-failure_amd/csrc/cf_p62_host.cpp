@@ -65,7 +65,11 @@ int log2_length62(int nm) {
     return lg;
 }
 
-// fact, ifact, inv, ifact2, mont (F1 each), then the forward and inverse twiddles (L each) of one prime.
+// Words of one prime's host-built tables: fact, ifact, inv, ifact2, mont (F1 each), the forward and
+// inverse twiddles (L each), then the roots w_16^r and w_16^-r (r < 8) as plain residues with their
+// Shoup quotients floor(v 2^64 / p) (32 words).
+size_t table_words62(int F1, int lg) { return 5 * (size_t)F1 + 2 * ((size_t)1 << lg) + 32; }
+
 void prime_tables(const Plan62& pl, Task62& td, u64* out) {
     const Mont62 mt(pl.p);
     const int F1 = pl.F1, L = 1 << pl.lg;
@@ -91,6 +95,12 @@ void prime_tables(const Plan62& pl, Task62& td, u64* out) {
             tw[h + j] = f, twi[h + j] = fi;
             f = mt.mul(f, s), fi = mt.mul(fi, si);
         }
+    }
+    u64* sh = twi + L;
+    for (int r = 0; r < 8; ++r) {
+        const u64 v = mt.mul(tw[8 + r], 1), vi = mt.mul(twi[8 + r], 1);  // out of Montgomery form
+        sh[r] = v, sh[8 + r] = (u64)(((u128)v << 64) / pl.p);
+        sh[16 + r] = vi, sh[24 + r] = (u64)(((u128)vi << 64) / pl.p);
     }
     td.m = mt.m;
     td.linv = mt.pow(mt.to((u64)L), pl.p - 2);  // 1/L, Montgomery form
@@ -187,7 +197,7 @@ int run_p62(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, 
                 qd.push_back(qq);
             }
             table_off[i] = table_words;
-            table_words += 5 * (size_t)pl.F1 + 2 * ((size_t)1 << pl.lg);
+            table_words += table_words62(pl.F1, pl.lg);
             arena += pl.words;
         }
         std::vector<u64> tables(table_words);
@@ -212,7 +222,7 @@ int run_p62(cf_engine* e, int d, int model, const int32_t* n, const int32_t* M, 
         int* dslots = (int*)((char*)dres + ((sizeof(u64) * qd.size() + 255) & ~(size_t)255));
         u64* mem = (u64*)e->arena;
         for (size_t i = 0; i < k; ++i) {
-            const size_t w = 5 * (size_t)plans[at + i].F1 + 2 * ((size_t)1 << plans[at + i].lg);
+            const size_t w = table_words62(plans[at + i].F1, plans[at + i].lg);
             HIP_TRY(e, hipMemcpyAsync(mem + td[i].arena, tables.data() + table_off[i], 8 * w, hipMemcpyHostToDevice, e->stream),
                     "upload");
         }
